@@ -1,0 +1,352 @@
+"""crystalgraphics_amd — MI355X (gfx950) spectral path-tracing core.
+
+Python face of the native libraries:
+  * the crystal::graphics API (EnvInit, EnvTerm, EnvPoll, EnvPresent,
+    EnvStatus, LoadScene, View, ViewToBuffer), as the reference's
+    include/CrystalGraphics headers declare it;
+  * HostScene / SceneData: procedural scenes and their flattened SceneData;
+  * DeviceEnv / DeviceScene: the C-ABI of include/cg_device.h (upload, the
+    per-pixel compute pass, batch closest-hit and occlusion queries).
+Every compute call runs on the GPU through libcg_device.so; there is no
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native
+from .native import (ALIAS, BLAS_NODE, EMISSIVE, HISTORY, HIT, INDEX, INSTANCE, MATERIAL, MISS_ID, MISS_T,
+                     PRIMITIVE, RAY, TLAS_NODE, VERTEX, Camera, Conf, NativeError)
+
+__all__ = ["HostScene", "SceneData", "DeviceEnv", "DeviceScene", "Camera", "Conf", "NativeError",
+           "EnvInit", "EnvTerm", "EnvPoll", "EnvPresent", "EnvStatus", "LoadScene", "View", "ViewToBuffer",
+           "MISS_ID", "MISS_T", "RAY", "HIT", "HISTORY", "VERTEX", "MATERIAL", "PRIMITIVE", "TLAS_NODE",
+           "INSTANCE", "BLAS_NODE", "INDEX", "EMISSIVE", "ALIAS"]
+
+
+def _host_check(rc: int) -> None:
+    if rc != 0:
+        raise NativeError(rc, native.host().cgh_last_error().decode())
+
+
+class SceneData:
+    """A flattened scene (the reference's SceneData) owned by the host library."""
+
+    def __init__(self, handle: int, scene: "HostScene"):
+        self._h = handle
+        self._scene = scene  # vertices and materials are referenced, keep alive
+        self._c = native.SceneDataC()
+        native.host().cgh_scene_data_view(self._h, C.byref(self._c))
+
+    @property
+    def c(self) -> native.SceneDataC:
+        return self._c
+
+    def arrays(self) -> dict:
+        return native.scene_data_arrays(self._c)
+
+    def close(self) -> None:
+        if self._h:
+            native.host().cgh_scene_data_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HostScene:
+    """A scene held by the host layer (the reference's Scene)."""
+
+    def __init__(self, handle: int):
+        self._h = handle
+
+    @classmethod
+    def generate(cls, name: str, seed: int = 1, params=()) -> "HostScene":
+        h = C.c_void_p()
+        p = np.asarray(params, dtype=np.float64)
+        rc = native.host().cgh_scene_generate(name.encode(), seed, p.ctypes.data if p.size else None,
+                                              int(p.size), C.byref(h))
+        _host_check(rc)
+        return cls(h.value)
+
+    @classmethod
+    def from_arrays(cls, vertices, indices, materials, prims, env=None, tag: int = 1) -> "HostScene":
+        v = np.ascontiguousarray(vertices, VERTEX)
+        i = np.ascontiguousarray(indices, np.uint32)
+        m = np.ascontiguousarray(materials, MATERIAL)
+        p = np.ascontiguousarray(prims, PRIMITIVE)
+        e = None if env is None else np.ascontiguousarray(env, np.float32)
+        h = C.c_void_p()
+        rc = native.host().cgh_scene_create(
+            v.ctypes.data, v.shape[0], i.ctypes.data, i.size, m.ctypes.data, m.shape[0], p.ctypes.data,
+            p.shape[0], None if e is None else e.ctypes.data, 0 if e is None else e.shape[1],
+            0 if e is None else e.shape[0], tag, C.byref(h))
+        _host_check(rc)
+        return cls(h.value)
+
+    def describe(self) -> dict:
+        d = native.SceneDescC()
+        native.host().cgh_scene_describe(self._h, C.byref(d))
+        out = {
+            "vertices": native.view_array(d.vertices, d.vertex_count, VERTEX),
+            "indices": native.view_array(d.indices, d.index_count, np.dtype("<u4")),
+            "materials": native.view_array(d.materials, d.material_count, MATERIAL),
+            "prims": native.view_array(d.prims, d.prim_count, PRIMITIVE),
+            "env": None if not d.env else native.view_array(d.env, d.env_w * d.env_h * 4, np.dtype("<f4")).reshape(
+                d.env_h, d.env_w, 4),
+            "camera": d.camera, "conf": d.conf, "tag": d.tag,
+        }
+        return out
+
+    def scene_data(self, share_blas: bool = True, threads: int = 8) -> SceneData:
+        h = C.c_void_p()
+        _host_check(native.host().cgh_scene_data_build(self._h, int(share_blas), int(threads), C.byref(h)))
+        return SceneData(h.value, self)
+
+    def rays(self, n: int, seed: int = 1) -> np.ndarray:
+        out = np.zeros(n, RAY)
+        _host_check(native.host().cgh_rays_generate(self._h, seed, n, out.ctypes.data))
+        return out
+
+    @property
+    def handle(self) -> int:
+        return self._h
+
+    def close(self) -> None:
+        if self._h:
+            native.host().cgh_scene_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceScene:
+    def __init__(self, env: "DeviceEnv", handle: int):
+        self.env = env
+        self._h = handle
+
+    @property
+    def handle(self) -> int:
+        return self._h
+
+    def footprint(self) -> tuple[int, int]:
+        b = C.c_uint64()
+        s = C.c_uint32()
+        self.env._check(native.device().cg_scene_footprint(self._h, C.byref(b), C.byref(s)))
+        return int(b.value), int(s.value)
+
+    def release(self) -> None:
+        if self._h and self.env._h:
+            self.env._check(native.device().cg_scene_release(self.env._h, self._h))
+        self._h = None
+
+
+class DeviceBuffer:
+    """An HBM allocation made through cg_device_alloc."""
+
+    def __init__(self, env: "DeviceEnv", nbytes: int):
+        self.env = env
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        env._check(native.device().cg_device_alloc(env._h, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        self.env._check(native.device().cg_memcpy_h2d(self.env._h, self.ptr, a.ctypes.data, a.nbytes))
+        return self
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.zeros(count, dtype)
+        assert out.nbytes <= self.nbytes
+        self.env._check(native.device().cg_memcpy_d2h(self.env._h, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            self.env._check(native.device().cg_device_free(self.env._h, self.ptr))
+            self.ptr = None
+
+
+class DeviceEnv:
+    """The HIP backend (cg_env): one stream, a history/target, scenes."""
+
+    def __init__(self, conf: Conf | None = None, device: int = 0):
+        self.conf = conf or Conf.make()
+        h = C.c_void_p()
+        rc = native.device().cg_env_create(C.byref(self.conf), device, C.byref(h))
+        if rc != 0:
+            raise NativeError(rc, native.device().cg_last_error().decode())
+        self._h = h.value
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            msg = native.device().cg_env_last_error(self._h).decode() if self._h else \
+                native.device().cg_last_error().decode()
+            raise NativeError(rc, msg)
+
+    @property
+    def handle(self) -> int:
+        return self._h
+
+    @property
+    def stream(self) -> int:
+        return native.device().cg_env_stream(self._h)
+
+    def upload(self, sd) -> DeviceScene:
+        """Upload a SceneData (or a dict of arrays in SceneData layout)."""
+        keep = None
+        if isinstance(sd, SceneData):
+            c = sd.c
+        else:
+            c, keep = native.make_scene_data(sd)
+        h = C.c_void_p()
+        self._check(native.device().cg_scene_upload(self._h, C.byref(c), C.byref(h)))
+        del keep
+        return DeviceScene(self, h.value)
+
+    def set_counting(self, on: bool = True) -> None:
+        self._check(native.device().cg_env_set_counting(self._h, int(on)))
+
+    def set_path_records(self, on: bool = True) -> None:
+        self._check(native.device().cg_env_set_path_records(self._h, int(on)))
+
+    def stats(self) -> dict:
+        s = native.Stats()
+        self._check(native.device().cg_stats_read(self._h, C.byref(s)))
+        return {k: int(getattr(s, k)) for k, _ in native.Stats._fields_}
+
+    def reset_stats(self) -> None:
+        self._check(native.device().cg_stats_reset(self._h))
+
+    def sync(self) -> None:
+        self._check(native.device().cg_env_sync(self._h))
+
+    def view(self, scene: DeviceScene, camera: Camera, iter_count: int = 0) -> None:
+        self._check(native.device().cg_view(self._h, scene.handle, C.byref(camera), iter_count))
+
+    def read_history(self) -> np.ndarray:
+        w, h = self.conf.render_width, self.conf.render_height
+        out = np.zeros(w * h, HISTORY)
+        self._check(native.device().cg_read_target(self._h, out.ctypes.data, None))
+        return out.reshape(h, w)
+
+    def read_rgba8(self) -> np.ndarray:
+        w, h = self.conf.render_width, self.conf.render_height
+        out = np.zeros((h, w, 4), np.uint8)
+        self._check(native.device().cg_read_target(self._h, None, out.ctypes.data))
+        return out
+
+    def read_path_records(self) -> tuple[np.ndarray, np.ndarray]:
+        w, h = self.conf.render_width, self.conf.render_height
+        hs = np.zeros(w * h, np.uint32)
+        rc = np.zeros(w * h, np.uint32)
+        self._check(native.device().cg_read_path_records(self._h, hs.ctypes.data, rc.ctypes.data))
+        return hs.reshape(h, w), rc.reshape(h, w)
+
+    def write_history(self, hist: np.ndarray) -> None:
+        a = np.ascontiguousarray(hist, HISTORY)
+        self._check(native.device().cg_write_history(self._h, a.ctypes.data))
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def trace_closest_device(self, scene: DeviceScene, rays: DeviceBuffer, n: int, hits: DeviceBuffer,
+                             uv: DeviceBuffer | None = None) -> None:
+        self._check(native.device().cg_trace_closest(self._h, scene.handle, rays.ptr, n, hits.ptr,
+                                                     uv.ptr if uv else None))
+
+    def trace_closest(self, scene: DeviceScene, rays: np.ndarray, with_uv: bool = False):
+        """Host convenience: copy rays in, closest hits (and uv) out."""
+        n = rays.shape[0]
+        rb = self.alloc(max(n, 1) * RAY.itemsize).upload(rays)
+        hb = self.alloc(max(n, 1) * HIT.itemsize)
+        ub = self.alloc(max(n, 1) * 8) if with_uv else None
+        try:
+            self.trace_closest_device(scene, rb, n, hb, ub)
+            hits = hb.download(HIT, n)
+            uv = ub.download(np.float32, 2 * n).reshape(n, 2) if ub else None
+        finally:
+            rb.free()
+            hb.free()
+            if ub:
+                ub.free()
+        return (hits, uv) if with_uv else hits
+
+    def trace_occluded(self, scene: DeviceScene, rays: np.ndarray, tmax: np.ndarray) -> np.ndarray:
+        n = rays.shape[0]
+        rb = self.alloc(max(n, 1) * RAY.itemsize).upload(rays)
+        tb = self.alloc(max(n, 1) * 4).upload(np.asarray(tmax, np.float32))
+        ob = self.alloc(max(n, 1) * 4)
+        try:
+            self._check(native.device().cg_trace_occluded(self._h, scene.handle, rb.ptr, tb.ptr, n, ob.ptr))
+            return ob.download(np.uint32, n)
+        finally:
+            rb.free()
+            tb.free()
+            ob.free()
+
+    def close(self) -> None:
+        if self._h:
+            native.device().cg_env_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# --- the crystal::graphics API (include/CrystalGraphics/*.h) ----------------
+def EnvInit(conf: Conf | None = None) -> None:
+    _host_check(native.host().cgh_env_init(C.byref(conf or Conf.make())))
+
+
+def EnvTerm() -> None:
+    _host_check(native.host().cgh_env_term())
+
+
+def EnvPoll() -> None:
+    _host_check(native.host().cgh_env_poll())
+
+
+def EnvPresent() -> None:
+    if not EnvStatus():
+        raise NativeError(5, "Environment not initialized.")
+
+
+def EnvStatus() -> bool:
+    return bool(native.host().cgh_env_status())
+
+
+def LoadScene(path: str) -> HostScene:
+    h = C.c_void_p()
+    _host_check(native.host().cgh_load_scene(str(path).encode(), C.byref(h)))
+    return HostScene(h.value)
+
+
+def View(scene: HostScene, camera: Camera | None = None) -> None:
+    _host_check(native.host().cgh_view(scene.handle, C.byref(camera or Camera.make((0, 0, 0), (1, 0, 0)))))
+
+
+def ViewToBuffer(scene: HostScene, camera: Camera, out: np.ndarray) -> np.ndarray:
+    assert out.dtype == np.float32 and out.flags.c_contiguous
+    _host_check(native.host().cgh_view_to_buffer(scene.handle, C.byref(camera), out.ctypes.data, out.size))
+    return out

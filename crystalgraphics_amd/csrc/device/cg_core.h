@@ -1,0 +1,860 @@
+// cg_core.h — device functions of the spectral path tracer.
+//
+// Float discipline: every expression is evaluated in the order the
+// reference's WGSL writes it, one IEEE binary32 operation at a time
+// (the library is compiled with -ffp-contract=off), min/max are the
+// NaN-ignoring fminf/fmaxf, f32->u32 conversions saturate, and runtime array
+// indices are clamped the way WebGPU robustness clamps them.  That makes
+// traversal, intersection, RNG, wavelength sampling, alias sampling, Smits
+// lifts and Fresnel terms bit-identical to a CPU restatement; only
+// cos/sin/tan/acos/atan2/exp/pow may differ in the last ulp.
+#ifndef CG_CORE_H_
+#define CG_CORE_H_
+
+#include <stdint.h>
+
+#include "../../../include/cg_scene.h"
+#include "cg_layout.h"
+
+#if defined(__HIPCC__) || defined(__HIP__)
+#include <hip/hip_runtime.h>
+#define CG_HD __host__ __device__ __forceinline__
+#define CG_SQRT sqrtf
+#define CG_FABS fabsf
+#else
+#include <cmath>
+#define CG_HD inline
+#define CG_SQRT std::sqrt
+#define CG_FABS std::fabs
+#endif
+
+#pragma clang fp contract(off)
+
+namespace cg {
+
+// ---------------------------------------------------------------------------
+// Vectors (evaluation order: left to right).
+struct f3 {
+  float x, y, z;
+};
+struct f2 {
+  float x, y;
+};
+CG_HD f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+CG_HD f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
+CG_HD f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+CG_HD f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+CG_HD f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+CG_HD f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+CG_HD f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
+CG_HD f3 operator*(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+CG_HD f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+CG_HD f2 operator+(f2 a, f2 b) { return f2{a.x + b.x, a.y + b.y}; }
+CG_HD f2 operator*(float s, f2 a) { return f2{s * a.x, s * a.y}; }
+CG_HD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+CG_HD f3 cross(f3 a, f3 b) {
+  return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+CG_HD float length(f3 a) { return CG_SQRT(dot(a, a)); }
+CG_HD f3 normalize(f3 a) {
+  float l = length(a);
+  return f3{a.x / l, a.y / l, a.z / l};
+}
+CG_HD float fmn(float a, float b) { return fminf(a, b); }
+CG_HD float fmx(float a, float b) { return fmaxf(a, b); }
+CG_HD float clampf(float x, float lo, float hi) { return fmn(fmx(x, lo), hi); }
+CG_HD f3 vmin(f3 a, f3 b) { return f3{fmn(a.x, b.x), fmn(a.y, b.y), fmn(a.z, b.z)}; }
+CG_HD f3 vmax(f3 a, f3 b) { return f3{fmx(a.x, b.x), fmx(a.y, b.y), fmx(a.z, b.z)}; }
+CG_HD uint32_t f2u(float x) {
+  if (!(x > 0.0f)) return 0u;
+  if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+  return static_cast<uint32_t>(x);
+}
+CG_HD uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+CG_HD uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+CG_HD f3 reflect(f3 e1, f3 e2) {
+  float s = 2.0f * dot(e2, e1);
+  return e1 - e2 * s;
+}
+CG_HD f3 refract(f3 e1, f3 e2, float e3) {
+  float d = dot(e2, e1);
+  float k = 1.0f - e3 * e3 * (1.0f - d * d);
+  if (k < 0.0f) return mk3(0.0f, 0.0f, 0.0f);
+  return e1 * e3 - e2 * (e3 * d + CG_SQRT(k));
+}
+CG_HD float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// Column-major 4x4: (M * vec4(p, 1)).xyz, (M * vec4(d, 0)).xyz and
+// (transpose(M) * vec4(n, 0)).xyz, summed column by column.
+CG_HD f3 mat_point(const float* m, f3 p) {
+  return ((ld3(m) * p.x + ld3(m + 4) * p.y) + ld3(m + 8) * p.z) + ld3(m + 12) * 1.0f;
+}
+CG_HD f3 mat_dir(const float* m, f3 d) {
+  return ((ld3(m) * d.x + ld3(m + 4) * d.y) + ld3(m + 8) * d.z) + ld3(m + 12) * 0.0f;
+}
+CG_HD f3 mat_t_dir(const float* m, f3 n) {
+  float r0 = ((m[0] * n.x + m[1] * n.y) + m[2] * n.z) + m[3] * 0.0f;
+  float r1 = ((m[4] * n.x + m[5] * n.y) + m[6] * n.z) + m[7] * 0.0f;
+  float r2 = ((m[8] * n.x + m[9] * n.y) + m[10] * n.z) + m[11] * 0.0f;
+  return mk3(r0, r1, r2);
+}
+
+// ---------------------------------------------------------------------------
+// Per-pixel RNG (shader/compute/math/rng.wesl:1-21).
+CG_HD void rng_step(uint32_t& s) {
+  s = s * 747796405u + 2891336453u;
+  uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+  s = (w >> 22u) ^ w;
+}
+CG_HD float rng_float(uint32_t& s) {
+  rng_step(s);
+  return static_cast<float>(s) / 4294967296.0f;
+}
+
+constexpr float kPi = 3.1415926535f;
+constexpr float kMax = 1e6f;
+
+// ---------------------------------------------------------------------------
+// Ray/box (math/geometric.wesl:5-26) and ray/triangle
+// (structural/bvh.wesl:188-221).
+CG_HD f3 dir_inv(f3 d) {
+  float x = d.x == 0.0f ? 1e-7f : d.x;
+  float y = d.y == 0.0f ? 1e-7f : d.y;
+  float z = d.z == 0.0f ? 1e-7f : d.z;
+  return mk3(1.0f / x, 1.0f / y, 1.0f / z);
+}
+// Slab test; returns t_near, or +inf when the reference's test rejects.
+CG_HD float box_enter(f3 o, f3 inv, const float* lo, const float* hi) {
+  float t0x = (lo[0] - o.x) * inv.x, t1x = (hi[0] - o.x) * inv.x;
+  float t0y = (lo[1] - o.y) * inv.y, t1y = (hi[1] - o.y) * inv.y;
+  float t0z = (lo[2] - o.z) * inv.z, t1z = (hi[2] - o.z) * inv.z;
+  float tn = fmx(fmx(fmn(t0x, t1x), fmn(t0y, t1y)), fmn(t0z, t1z));
+  float tf = fmn(fmn(fmx(t0x, t1x), fmx(t0y, t1y)), fmx(t0z, t1z));
+  return (tf >= tn && tf > 0.0f) ? tn : __builtin_inff();
+}
+struct TriT {
+  float t, u, v;
+};
+// Moller-Trumbore with the edges precomputed (e1 = v1 - v0, e2 = v2 - v0 are
+// the same subtractions the reference performs per test).
+CG_HD TriT tri_test(f3 o, f3 d, f3 v0, f3 e1, f3 e2) {
+  TriT r{-1.0f, 0.0f, 0.0f};
+  f3 h = cross(d, e2);
+  float a = dot(e1, h);
+  if (a > -1e-6f && a < 1e-6f) return r;
+  float f = 1.0f / a;
+  f3 s = o - v0;
+  float u = f * dot(s, h);
+  if (u < 0.0f || u > 1.0f) return r;
+  f3 q = cross(s, e1);
+  float v = f * dot(d, q);
+  if (v < 0.0f || u + v > 1.0f) return r;
+  float t = f * dot(e2, q);
+  if (t > 1e-6f) {
+    r.t = t;
+    r.u = u;
+    r.v = v;
+  }
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Scene view used by the kernels.
+struct DevScene {
+  const PairNode* nodes;
+  const TriRec* tris;
+  const InstRec* insts;
+  const uint4* tri_vidx;   // per slot: source vertex indices
+  const VAttr* vattr;
+  const MatRec* mats;
+  uint32_t mat_count;
+  const EmitRec* emit;
+  uint32_t emit_count;
+  const float2* alias;     // {bitcast alias index, prob}
+  uint32_t alias_count;
+  const float4* env;
+  uint32_t env_w, env_h;
+  uint32_t root;           // TLAS root ref
+};
+
+struct TravStats {
+  uint32_t nodes, tris, insts;
+};
+
+struct HitRec {
+  float t, u, v;
+  uint32_t inst, slot;
+};
+
+// Two-level closest-hit (or any-hit) traversal over 64-byte pair nodes:
+// near child first, far child pushed, boxes entered beyond the best hit so far
+// skipped, a marker on the stack to return from a BLAS to the TLAS.  The
+// closest hit equals the reference's unordered walk (bvh.wesl:56-186) except
+// between triangles hit at exactly the same distance.  `stk` holds `stride`-
+// interleaved entries (LDS: stride = workgroup size).
+template <bool kAnyHit>
+CG_HD HitRec traverse(const DevScene& S, f3 wo, f3 wd, float tmax, uint32_t* stk, int stride,
+                      TravStats& st) {
+  HitRec best{tmax, 0.0f, 0.0f, CG_MISS_ID, CG_MISS_ID};
+  uint32_t ref = S.root;
+  if (ref == CG_REF_NONE) return best;
+  const f3 winv = dir_inv(wd);
+  f3 o = wo, d = wd, inv = winv;
+  uint32_t cur_inst = CG_MISS_ID;
+  int sp = 0;
+  for (;;) {
+    uint32_t kind = ref & CG_REF_KIND_MASK;
+    if (kind == 0u) {
+      const PairNode* n = S.nodes + ref;
+      st.nodes++;
+      float tl = box_enter(o, inv, n->lo_l, n->hi_l);
+      float tr = box_enter(o, inv, n->lo_r, n->hi_r);
+      bool hl = tl <= best.t, hr = tr <= best.t;
+      if (hl && hr) {
+        bool left_first = tl <= tr;
+        stk[sp * stride] = left_first ? n->ref_r : n->ref_l;
+        ++sp;
+        ref = left_first ? n->ref_l : n->ref_r;
+        continue;
+      }
+      if (hl) {
+        ref = n->ref_l;
+        continue;
+      }
+      if (hr) {
+        ref = n->ref_r;
+        continue;
+      }
+    } else if (kind == CG_REF_INST) {
+      uint32_t ii = ref & CG_REF_INDEX_MASK;
+      const InstRec* in = S.insts + ii;
+      st.insts++;
+      if (in->root != CG_REF_NONE) {
+        o = mat_point(in->inv, wo);
+        d = mat_dir(in->inv, wd);
+        inv = dir_inv(d);
+        cur_inst = ii;
+        stk[sp * stride] = CG_REF_MARKER;
+        ++sp;
+        ref = in->root;
+        continue;
+      }
+    } else {
+      uint32_t slot = ref & CG_REF_INDEX_MASK & ~CG_REF_LEAF2;
+      uint32_t cnt = (ref & CG_REF_LEAF2) ? 2u : 1u;
+      for (uint32_t i = 0; i < cnt; ++i) {
+        const TriRec* tr = S.tris + slot + i;
+        st.tris++;
+        TriT h = tri_test(o, d, ld3(tr->v0), ld3(tr->e1), ld3(tr->e2));
+        if (h.t > 0.0f && h.t < best.t) {
+          best.t = h.t;
+          best.u = h.u;
+          best.v = h.v;
+          best.inst = cur_inst;
+          best.slot = slot + i;
+          if (kAnyHit) return best;
+        }
+      }
+    }
+    // pop
+    for (;;) {
+      if (sp == 0) return best;
+      --sp;
+      ref = stk[sp * stride];
+      if (ref != CG_REF_MARKER) break;
+      o = wo;
+      d = wd;
+      inv = winv;
+      cur_inst = CG_MISS_ID;
+    }
+  }
+}
+
+// Surface at a hit (bvh.wesl:103-129 and the per-hit attribute fetch of
+// bvh.wesl:164-180), computed once for the final closest hit.
+struct Surface {
+  f3 pos, sn, gn;
+  uint32_t material;
+  f2 uv0, uv1;
+};
+CG_HD Surface surface_at(const DevScene& S, f3 wo, f3 wd, const HitRec& h) {
+  const InstRec* in = S.insts + h.inst;
+  const TriRec* tr = S.tris + h.slot;
+  uint4 vi = S.tri_vidx[h.slot];
+  const VAttr& a0 = S.vattr[vi.x];
+  const VAttr& a1 = S.vattr[vi.y];
+  const VAttr& a2 = S.vattr[vi.z];
+  float w0 = 1.0f - h.u - h.v;
+  f3 ln = normalize((w0 * ld3(a0.nrm) + h.u * ld3(a1.nrm)) + h.v * ld3(a2.nrm));
+  f3 lgn = normalize(cross(ld3(tr->e1), ld3(tr->e2)));
+  f2 u00{a0.uv0[0], a0.uv0[1]}, u01{a1.uv0[0], a1.uv0[1]}, u02{a2.uv0[0], a2.uv0[1]};
+  f2 u10{a0.uv1[0], a0.uv1[1]}, u11{a1.uv1[0], a1.uv1[1]}, u12{a2.uv1[0], a2.uv1[1]};
+  Surface s;
+  f3 wgn = normalize(mat_t_dir(in->inv, lgn));
+  f3 wsn = normalize(mat_t_dir(in->inv, ln));
+  s.pos = wo + h.t * wd;
+  s.sn = dot(wsn, wgn) >= 0.0f ? wsn : -wsn;
+  s.gn = wgn;
+  s.material = in->material;
+  s.uv0 = (w0 * u00 + h.u * u01) + h.v * u02;
+  s.uv1 = (w0 * u10 + h.u * u11) + h.v * u12;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Spectral pieces.  Smits (1999) basis as the reference tabulates it
+// (shading/smits.wesl:3-124): bin = u32(10 * (l - 380) / 340) with the upper
+// neighbour clamped to the last sample.
+struct SmitsTables {
+  float bins[10];
+  float curve[7][10];
+};
+#if defined(__HIPCC__) || defined(__HIP__)
+__constant__
+#endif
+static const SmitsTables kSmits = {
+    {380.0f, 417.7778f, 455.5556f, 493.3333f, 531.1111f, 568.8889f, 606.6667f, 644.4444f, 682.2222f, 720.0f},
+    {{1.0f, 1.0f, 0.9999f, 0.9993f, 0.9992f, 0.9998f, 1.0f, 1.0f, 1.0f, 1.0f},
+     {0.1012f, 0.0515f, 0.0f, 0.0f, 0.0f, 0.0f, 0.8325f, 1.0149f, 1.0149f, 1.0149f},
+     {0.0f, 0.0f, 0.0273f, 0.7937f, 1.0f, 0.9418f, 0.1719f, 0.0f, 0.0f, 0.0025f},
+     {1.0f, 1.0f, 0.8916f, 0.3323f, 0.0f, 0.0f, 0.0003f, 0.0369f, 0.0483f, 0.0496f},
+     {1.0f, 1.0f, 0.9685f, 0.2229f, 0.0f, 0.0458f, 0.8369f, 1.0f, 1.0f, 0.9959f},
+     {0.971f, 0.9426f, 1.0007f, 1.0007f, 1.0007f, 1.0007f, 0.1564f, 0.0f, 0.0f, 0.0f},
+     {0.0001f, 0.0f, 0.1088f, 0.6651f, 1.0f, 1.0f, 0.9996f, 0.9586f, 0.9685f, 0.984f}}};
+enum { kWhite = 0, kRed, kGreen, kBlue, kMagenta, kCyan, kYellow };
+
+// The seven curves share the wavelength's bin and mix weight.
+struct SmitsBin {
+  uint32_t b0, b1;
+  float mix;
+  int edge;  // -1: l <= 380, +1: l >= 720, 0 interior
+};
+CG_HD SmitsBin smits_bin(float l) {
+  SmitsBin b{0u, 0u, 0.0f, 0};
+  if (l <= 380.0f) {
+    b.edge = -1;
+    return b;
+  }
+  if (l >= 720.0f) {
+    b.edge = 1;
+    return b;
+  }
+  uint32_t bin = f2u(10.0f * (l - 380.0f) / 340.0f);
+  b.b0 = umin(bin, 9u);
+  b.b1 = umin(bin + 1u, 9u);
+  b.mix = (l - kSmits.bins[b.b0]) / (kSmits.bins[b.b1] - kSmits.bins[b.b0]);
+  return b;
+}
+CG_HD float smits_at(int c, const SmitsBin& b) {
+  if (b.edge < 0) return kSmits.curve[c][0];
+  if (b.edge > 0) return kSmits.curve[c][9];
+  return (1.0f - b.mix) * kSmits.curve[c][b.b0] + b.mix * kSmits.curve[c][b.b1];
+}
+// material.wesl:264-297 / environment.wesl:27-61 / pdf/emission.wesl:84-117.
+CG_HD float rgb_lift(f3 rgb, const SmitsBin& b) {
+  float s = 0.0f;
+  float r = rgb.x, g = rgb.y, bl = rgb.z;
+  if (r <= g && g <= bl) {
+    s += r * smits_at(kWhite, b);
+    s += (g - r) * smits_at(kCyan, b);
+    s += (bl - g) * smits_at(kBlue, b);
+  } else if (g <= r && g <= bl) {
+    s += g * smits_at(kWhite, b);
+    if (r <= bl) {
+      s += (r - g) * smits_at(kMagenta, b);
+      s += (bl - r) * smits_at(kBlue, b);
+    } else {
+      s += (bl - g) * smits_at(kMagenta, b);
+      s += (r - bl) * smits_at(kRed, b);
+    }
+  } else {
+    s += bl * smits_at(kWhite, b);
+    if (r <= g) {
+      s += (r - bl) * smits_at(kYellow, b);
+      s += (g - r) * smits_at(kGreen, b);
+    } else {
+      s += (g - bl) * smits_at(kYellow, b);
+      s += (r - g) * smits_at(kRed, b);
+    }
+  }
+  return s;
+}
+CG_HD float rgb_reflectance(f3 rgb, const SmitsBin& b) { return clampf(0.94f * rgb_lift(rgb, b), 0.0f, 1.0f); }
+CG_HD float rgb_radiance(f3 rgb, const SmitsBin& b) { return fmx(rgb_lift(rgb, b), 0.0f); }
+
+// ---------------------------------------------------------------------------
+// Material at one wavelength (shading/material.wesl:19-110).
+struct IOR {
+  float re, im;
+};
+struct Mat {
+  float diffuse, alpha;
+  IOR ior;
+  float transmission;
+  bool volume;
+  float emission;
+};
+CG_HD float dispersed_ior(float nd, float disp, float l) {
+  if (disp <= 0.0f) return nd;
+  float inv_abbe = disp / 20.0f;
+  return fmx(nd + (nd - 1.0f) * inv_abbe * (523655.0f / (l * l) - 1.5168f), 1.0f);
+}
+CG_HD Mat decode(const MatRec& m, float l, const SmitsBin& b) {
+  float refl = rgb_reflectance(ld3(m.base), b);
+  float real_ior = (m.metallic == 0.0f && m.transmission > 0.0f) ? dispersed_ior(m.ior, m.dispersion, l) : m.ior;
+  float np1 = real_ior + 1.0f;
+  float im = CG_SQRT(clampf((4.0f * real_ior) / (1.0f - refl) - np1 * np1, 0.0f, 10.0f));
+  Mat r;
+  r.diffuse = (1.0f - m.metallic) * (1.0f - m.transmission) * refl;
+  r.alpha = m.alpha;
+  r.ior = IOR{real_ior, m.metallic * im};
+  r.transmission = m.transmission;
+  r.volume = m.volume != 0u;
+  r.emission = rgb_reflectance(ld3(m.emission), b);
+  return r;
+}
+
+// shading/util.wesl:4-81
+CG_HD f3 oriented(f3 d, f3 n) { return dot(d, n) > 0.0f ? -n : n; }
+CG_HD IOR inv_ior(IOR i) {
+  float nsq = i.re * i.re + i.im * i.im;
+  return IOR{i.re / nsq, -i.im / nsq};
+}
+CG_HD IOR rel_ior(f3 sn, f3 d, const Mat& m) { return dot(sn, d) < 0.0f ? inv_ior(m.ior) : m.ior; }
+CG_HD float fresnel(f3 d, f3 n, IOR rel) {
+  float ci = clampf(dot(-d, n), 0.0f, 1.0f);
+  if (CG_FABS(rel.im) > 1e-6f) {
+    float re = rel.re, im = rel.im;
+    float rsq = re * re, isq = im * im, csq = ci * ci;
+    float nsq = rsq + isq;
+    float trc = 2.0f * re * ci;
+    float perp = (nsq - trc + csq) / fmx(nsq + trc + csq, 1e-6f);
+    float par = (nsq * csq - trc + 1.0f) / fmx(nsq * csq + trc + 1.0f, 1e-6f);
+    return clampf(0.5f * (perp + par), 0.0f, 1.0f);
+  }
+  float r = rel.re;
+  float ts2 = r * r * fmx(0.0f, 1.0f - ci * ci);
+  if (ts2 >= 1.0f) return 1.0f;
+  float tc = CG_SQRT(fmx(0.0f, 1.0f - ts2));
+  float perp = (r * ci - tc) / (r * ci + tc);
+  float par = (ci - r * tc) / (ci + r * tc);
+  return clampf(0.5f * (perp * perp + par * par), 0.0f, 1.0f);
+}
+
+// math/geometric.wesl:28-33
+struct Frame {
+  f3 t, b, n;
+};
+CG_HD Frame tangent_space(f3 n) {
+  f3 helper = CG_FABS(n.x) > 0.9f ? mk3(0.0f, 1.0f, 0.0f) : mk3(1.0f, 0.0f, 0.0f);
+  f3 t = normalize(cross(helper, n));
+  return Frame{t, cross(n, t), n};
+}
+CG_HD f3 to_world(const Frame& f, f3 v) { return (f.t * v.x + f.b * v.y) + f.n * v.z; }
+CG_HD f3 to_local(const Frame& f, f3 v) { return mk3(dot(f.t, v), dot(f.b, v), dot(f.n, v)); }
+
+// ---------------------------------------------------------------------------
+// Lobes (shading/pdf/*.wesl, shading/dist/*.wesl).
+struct TSample {
+  f3 dir;
+  float prob;
+};
+CG_HD float diffuse_pdf(f3 d, f3 next, const Surface& s) {
+  f3 macro = oriented(d, s.sn);
+  return fmx(dot(next, macro), 0.0f) / kPi;
+}
+CG_HD float diffuse_dist(f3 d, f3 next, const Surface& s, const Mat& m) {
+  f3 macro = oriented(d, s.sn);
+  float np = fmx(dot(next, macro), 0.0f);
+  return m.diffuse * np / kPi;
+}
+CG_HD TSample diffuse_sample(f3 d, const Surface& s, uint32_t& rng) {
+  f3 macro = oriented(d, s.sn);
+  float r = CG_SQRT(rng_float(rng));
+  float angle = 2.0f * kPi * rng_float(rng);
+  f3 local = mk3(r * cosf(angle), r * sinf(angle), CG_SQRT(fmx(0.0f, 1.0f - r * r)));
+  f3 next = to_world(tangent_space(macro), local);
+  return TSample{next, fmx(dot(next, macro), 0.0f) / kPi};
+}
+CG_HD f3 specular_dir(f3 d, const Surface& s) { return reflect(d, oriented(d, s.sn)); }
+CG_HD float spec_reflect_pdf(f3 d, f3 next, const Surface& s) {
+  f3 df = next - specular_dir(d, s);
+  return dot(df, df) <= 1e-6f ? 1.0f : 0.0f;
+}
+CG_HD float spec_reflect_dist(f3 d, f3 next, const Surface& s, const Mat& m) {
+  f3 macro = oriented(d, s.sn);
+  f3 df = next - reflect(d, macro);
+  float distribution = dot(df, df) <= 1e-6f ? 1.0f : 0.0f;
+  return fresnel(d, macro, rel_ior(s.sn, d, m)) * distribution;
+}
+CG_HD f3 spec_trans_dir(f3 d, const Surface& s, const Mat& m) {
+  if (m.volume) return refract(d, oriented(d, s.sn), rel_ior(s.sn, d, m).re);
+  return d;
+}
+CG_HD float spec_trans_pdf(f3 d, f3 next, const Surface& s, const Mat& m) {
+  f3 e = spec_trans_dir(d, s, m);
+  f3 df = next - e;
+  return (dot(e, e) > 1e-6f && dot(df, df) <= 1e-6f) ? 1.0f : 0.0f;
+}
+CG_HD float spec_trans_dist(f3 d, f3 next, const Surface& s, const Mat& m) {
+  f3 macro = oriented(d, s.sn);
+  f3 df = next - spec_trans_dir(d, s, m);
+  float distribution = dot(df, df) <= 1e-6f ? 1.0f : 0.0f;
+  return m.transmission * (1.0f - fresnel(d, macro, rel_ior(s.sn, d, m))) * distribution;
+}
+// GGX with the reference's alpha (not alpha^2) in D and Lambda
+// (pdf/rough_reflection.wesl:120-139).
+CG_HD float ggx_d(f3 micro, f3 macro, float alpha) {
+  float c = clampf(dot(micro, macro), 0.0f, 1.0f);
+  float c2 = c * c;
+  float t2 = fmx(0.0f, 1.0f - c2) / fmx(c2, 1e-6f);
+  float at = alpha + t2;
+  float den = kPi * (c2 * c2) * (at * at);
+  return alpha / fmx(den, 1e-6f);
+}
+CG_HD float ggx_g1(float angle, float alpha) {
+  float t = tanf(angle);
+  return 1.0f / (1.0f + (CG_SQRT(1.0f + alpha * t * t) - 1.0f) / 2.0f);
+}
+CG_HD float ggx_g2(float a1, float a2, float alpha) {
+  float t1 = tanf(a1), t2 = tanf(a2);
+  return 1.0f / (1.0f + (CG_SQRT(1.0f + alpha * t1 * t1) - 1.0f) / 2.0f +
+                 (CG_SQRT(1.0f + alpha * t2 * t2) - 1.0f) / 2.0f);
+}
+CG_HD f3 sample_micro_normal(f3 d, const Surface& s, float alpha, uint32_t& rng) {
+  f3 macro = oriented(d, s.sn);
+  Frame f = tangent_space(macro);
+  f3 local = normalize(to_local(f, -d));
+  f3 st = normalize(mk3(alpha * local.x, alpha * local.y, fmx(local.z, 1e-6f)));
+  float tl2 = st.x * st.x + st.y * st.y;
+  f3 tx = tl2 > 1e-6f ? mk3(-st.y, st.x, 0.0f) / CG_SQRT(fmx(tl2, 1e-6f)) : mk3(1.0f, 0.0f, 0.0f);
+  f3 ty = cross(st, tx);
+  float radius = CG_SQRT(rng_float(rng));
+  float angle = 2.0f * kPi * rng_float(rng);
+  float px = radius * cosf(angle);
+  float py = radius * sinf(angle);
+  float vis = 0.5f * (1.0f + st.z);
+  py = mixf(CG_SQRT(fmx(0.0f, 1.0f - px * px)), py, vis);
+  f3 lsm = px * tx + py * ty + CG_SQRT(fmx(0.0f, 1.0f - px * px - py * py)) * st;
+  f3 lm = normalize(mk3(alpha * lsm.x, alpha * lsm.y, fmx(lsm.z, 1e-6f)));
+  return normalize(to_world(f, lm));
+}
+CG_HD float reflect_pdf(f3 d, f3 micro, f3 macro, float alpha) {
+  float cmp = fmx(dot(-d, macro), 1e-6f);
+  float cmi = fmx(dot(-d, micro), 1e-6f);
+  float vnp = ggx_d(micro, macro, alpha) * ggx_g1(acosf(cmp), alpha) * cmi / cmp;
+  return vnp / (4.0f * cmi);
+}
+CG_HD float rough_reflect_pdf(f3 d, f3 next, const Surface& s, const Mat& m) {
+  f3 macro = oriented(d, s.sn);
+  if (dot(next, macro) <= 0.0f) return 0.0f;
+  f3 mu = next - d;
+  if (dot(mu, mu) <= 1e-6f) return 0.0f;
+  return reflect_pdf(d, oriented(d, normalize(mu)), macro, m.alpha);
+}
+CG_HD TSample rough_reflect_sample(f3 d, const Surface& s, const Mat& m, uint32_t& rng) {
+  f3 micro = oriented(d, sample_micro_normal(d, s, m.alpha, rng));
+  f3 next = reflect(d, micro);
+  f3 macro = oriented(d, s.sn);
+  if (dot(next, macro) <= 0.0f) return TSample{next, 0.0f};
+  return TSample{next, reflect_pdf(d, micro, macro, m.alpha)};
+}
+CG_HD float rough_reflect_dist(f3 d, f3 next, const Surface& s, const Mat& m) {
+  f3 macro = oriented(d, s.sn);
+  float np = fmx(dot(next, macro), 0.0f);
+  if (np <= 0.0f) return 0.0f;
+  f3 mu = next - d;
+  if (dot(mu, mu) <= 1e-6f) return 0.0f;
+  f3 raw = normalize(mu);
+  f3 micro = oriented(d, raw);
+  IOR rel = rel_ior(raw, d, m);
+  float cmp = fmx(dot(-d, macro), 1e-6f);
+  float nmp = fmx(dot(next, macro), 1e-6f);
+  float rd = ggx_d(micro, macro, m.alpha) * ggx_g2(acosf(cmp), acosf(nmp), m.alpha) / (4.0f * cmp * nmp);
+  return fresnel(d, micro, rel) * rd * np;
+}
+CG_HD float rel_value(IOR i) { return fmx(i.re, 1e-6f); }
+CG_HD float transmit_pdf(f3 d, f3 micro, f3 macro, f3 next, float tior, float alpha) {
+  float cmp = fmx(dot(-d, macro), 1e-6f);
+  float cmi = fmx(dot(-d, micro), 1e-6f);
+  float vnp = ggx_d(micro, macro, alpha) * ggx_g1(acosf(cmp), alpha) * cmi / cmp;
+  float nmi = dot(next, micro);
+  float den = nmi + cmi / tior;
+  return vnp * cmi / fmx(den * den, 1e-6f);
+}
+CG_HD float rough_trans_pdf(f3 d, f3 next, const Surface& s, const Mat& m) {
+  f3 macro = oriented(d, s.sn);
+  if (dot(next, macro) >= 0.0f) return 0.0f;
+  float mrv = rel_value(rel_ior(macro, d, m));
+  f3 mu = mrv * d - next;
+  if (dot(mu, mu) <= 1e-6f) return 0.0f;
+  f3 micro = oriented(d, normalize(mu));
+  float tior = 1.0f / rel_value(rel_ior(micro, d, m));
+  return transmit_pdf(d, micro, macro, next, tior, m.alpha);
+}
+CG_HD TSample rough_trans_sample(f3 d, const Surface& s, const Mat& m, uint32_t& rng) {
+  f3 macro = oriented(d, s.sn);
+  f3 micro = oriented(d, sample_micro_normal(d, s, m.alpha, rng));
+  f3 next = refract(d, micro, rel_value(rel_ior(micro, d, m)));
+  if (dot(next, next) <= 1e-6f) return TSample{next, 0.0f};
+  if (dot(next, macro) >= 0.0f) return TSample{next, 0.0f};
+  return TSample{next, rough_trans_pdf(d, next, s, m)};
+}
+CG_HD float rough_trans_dist(f3 d, f3 next, const Surface& s, const Mat& m) {
+  f3 macro = oriented(d, s.sn);
+  if (dot(next, macro) >= 0.0f) return 0.0f;
+  float np = fmx(dot(-next, macro), 0.0f);
+  float mrv = rel_value(rel_ior(macro, d, m));
+  f3 mu = mrv * d - next;
+  if (dot(mu, mu) <= 1e-6f) return 0.0f;
+  f3 raw = normalize(mu);
+  f3 micro = oriented(d, raw);
+  IOR rel = rel_ior(raw, d, m);
+  float tior = 1.0f / rel_value(rel);
+  float cmp = fmx(dot(-d, macro), 1e-6f);
+  float nmp = fmx(dot(-next, macro), 1e-6f);
+  float cmi = fmx(dot(-d, micro), 1e-6f);
+  float nmi = dot(next, micro);
+  float den = nmi + cmi / tior;
+  float td = m.transmission * ggx_d(micro, macro, m.alpha) * ggx_g2(acosf(cmp), acosf(nmp), m.alpha) *
+             CG_FABS(cmi * nmi) / fmx(cmp * nmp * den * den, 1e-6f);
+  return (1.0f - fresnel(d, micro, rel)) * td * np;
+}
+
+// ---------------------------------------------------------------------------
+// Emitters (shading/pdf/emission.wesl:28-191).
+CG_HD float emit_hit_dist(f3 pos, f3 dir, const EmitRec& e) {
+  f3 p0 = ld3(e.v0);
+  f3 e1 = ld3(e.v1) - p0, e2 = ld3(e.v2) - p0;
+  f3 pv = cross(dir, e2);
+  float det = dot(e1, pv);
+  if (CG_FABS(det) <= 1e-3f) return -1.0f;
+  float inv = 1.0f / det;
+  f3 tv = pos - p0;
+  float u = dot(tv, pv) * inv;
+  if (u < 0.0f || u > 1.0f) return -1.0f;
+  f3 qv = cross(tv, e1);
+  float v = dot(dir, qv) * inv;
+  if (v < 0.0f || u + v > 1.0f) return -1.0f;
+  float hd = dot(e2, qv) * inv;
+  return hd > 1e-3f ? hd : -1.0f;
+}
+// Solid-angle pdf of the emitters the ray from `pos` along `next` passes
+// through, summed in emitter order (emission.wesl:120-149).
+CG_HD float light_pdf(const DevScene& S, f3 next, f3 pos) {
+  if (S.alias_count == 0u) return 0.0f;
+  float pdf = 0.0f;
+  f3 rd = normalize(next);
+  for (uint32_t i = 0; i < S.emit_count; ++i) {
+    const EmitRec& e = S.emit[i];
+    float hd = emit_hit_dist(pos, rd, e);
+    if (hd <= 0.0f) continue;
+    f3 ln = mk3(e.ln_x, e.ln_y, e.ln_z);
+    float lp = CG_FABS(dot(ln, -rd));
+    pdf += e.area_pdf * hd * hd / fmx(lp, 1e-3f);
+  }
+  return pdf;
+}
+struct ESample {
+  f3 dir;
+  float prob, radiance, dist;
+};
+CG_HD ESample emission_sample(const DevScene& S, const Surface& s, const SmitsBin& b, uint32_t& rng) {
+  uint32_t n = S.alias_count;
+  uint32_t bin = f2u(rng_float(rng) * static_cast<float>(n));
+  float randf = rng_float(rng);
+  uint32_t cb = umin(bin, n - 1u);
+  float2 ent = S.alias[cb];
+  uint32_t pick = randf < ent.y ? bin : __builtin_bit_cast(uint32_t, ent.x);
+  const EmitRec& e = S.emit[umin(pick, S.emit_count - 1u)];
+  float tx = rng_float(rng);
+  float ty = rng_float(rng);
+  float sq = CG_SQRT(tx);
+  float bx = 1.0f - sq, by = sq * (1.0f - ty), bz = sq * ty;
+  f3 sp = (bx * ld3(e.v0) + by * ld3(e.v1)) + bz * ld3(e.v2);
+  f3 off = sp - s.pos;
+  float dsq = fmx(dot(off, off), 1e-3f);
+  f3 next = off / CG_SQRT(dsq);
+  f3 ln = mk3(e.ln_x, e.ln_y, e.ln_z);
+  float lp = CG_FABS(dot(ln, -next));
+  float sapdf = e.area_pdf * dsq / fmx(lp, 1e-3f);
+  return ESample{next, sapdf, rgb_radiance(ld3(e.rgb), b), CG_SQRT(dsq)};
+}
+
+// ---------------------------------------------------------------------------
+// Lobe counts and balance-heuristic MIS (shading/sample.wesl:110-501).
+struct Counts6 {
+  uint32_t d, sr, rr, st, rt, l;
+};
+struct TraceParams {
+  int D;            // lod MAX_DEPTH == TRACE_DEPTH (clamped to [1, 64])
+  uint32_t T, E, DIF, ROUGH;
+};
+CG_HD uint32_t scaled_count(float h, float scale, uint32_t mx) {
+  if (h <= 1e-6f) return 0u;
+  return umin(umax(1u, f2u(h * scale)), mx);
+}
+CG_HD Counts6 lobe_counts(const Mat& m, bool can_light, int depth, const TraceParams& P) {
+  float lo = m.ior.re - 1.0f, hi = m.ior.re + 1.0f, isq = m.ior.im * m.ior.im;
+  float refl = clampf((lo * lo + isq) / fmx(hi * hi + isq, 1e-6f), 0.0f, 1.0f);
+  float hd = m.diffuse, trans = m.transmission;
+  float hsr = m.alpha == 0.0f ? refl : 0.0f;
+  float hrr = m.alpha > 0.0f ? refl : 0.0f;
+  float hst = (trans > 1e-6f && m.alpha == 0.0f) ? trans : 0.0f;
+  float hrt = (trans > 1e-6f && m.alpha > 0.0f) ? trans : 0.0f;
+  float hl = can_light ? 1.0f : 0.0f;
+  float heur = static_cast<float>(P.D - depth) / static_cast<float>(P.D);
+  uint32_t t_avail = f2u(heur * static_cast<float>(umin(P.T, 4u)));
+  uint32_t e_avail = f2u(heur * static_cast<float>(umin(P.E, 1u)));
+  Counts6 c{0u, 0u, 0u, 0u, 0u, 0u};
+  float sum = hd + hsr + hrr + hst + hrt;
+  if (!(t_avail == 0u || sum <= 1e-6f)) {
+    float scale = static_cast<float>(t_avail) / sum;
+    uint32_t rem = t_avail;
+    c.d = umin(scaled_count(hd, scale, umin(P.DIF, 1u)), rem);
+    rem -= c.d;
+    c.sr = umin(scaled_count(hsr, scale, 1u), rem);
+    rem -= c.sr;
+    c.rr = umin(scaled_count(hrr, scale, umin(P.ROUGH, 1u)), rem);
+    rem -= c.rr;
+    c.st = umin(scaled_count(hst, scale, 1u), rem);
+    rem -= c.st;
+    c.rt = umin(scaled_count(hrt, scale, umin(P.ROUGH, 1u)), rem);
+  }
+  c.l = (hl > 1e-6f && e_avail > 0u) ? umax(1u, e_avail) : 0u;
+  return c;
+}
+enum { kNone = 0, kDiffuse, kSpecR, kRoughR, kSpecT, kRoughT };
+// The light pdf multiplies a zero count everywhere below the first bounce;
+// it is finite for finite emitters, so it is only evaluated when counted.
+CG_HD float balance_den(const DevScene& S, f3 d, f3 next, const Surface& s, const Mat& m, const Counts6& c,
+                        int model, float model_pdf, float light) {
+  float pd = model == kDiffuse ? model_pdf : diffuse_pdf(d, next, s);
+  float psr = model == kSpecR ? model_pdf : spec_reflect_pdf(d, next, s);
+  float prr = model == kRoughR ? model_pdf : rough_reflect_pdf(d, next, s, m);
+  float pst = model == kSpecT ? model_pdf : spec_trans_pdf(d, next, s, m);
+  float prt = model == kRoughT ? model_pdf : rough_trans_pdf(d, next, s, m);
+  float pl = light >= 0.0f ? light : (c.l != 0u ? light_pdf(S, next, s.pos) : 0.0f);
+  return fmx(static_cast<float>(c.d) * pd + static_cast<float>(c.sr) * psr + static_cast<float>(c.rr) * prr +
+                 static_cast<float>(c.st) * pst + static_cast<float>(c.rt) * prt + static_cast<float>(c.l) * pl,
+             1e-6f);
+}
+
+// One shading frame of the depth-first transport tree (shading.wesl:24-53).
+struct Frame5 {
+  f3 pos, gn;
+  f3 tdir[4];
+  float tcoeff[4];
+  f3 edir;
+  float erad, edist, emission, acc;
+  uint32_t tcount, ecount, iter;
+};
+CG_HD void forward_pass(const DevScene& S, f3 d, float l, const SmitsBin& b, const Surface& s, int depth,
+                        const TraceParams& P, uint32_t& rng, Frame5& x) {
+  const MatRec& pm = S.mats[umin(s.material, S.mat_count - 1u)];
+  Mat m = decode(pm, l, b);
+  Counts6 c = lobe_counts(m, S.alias_count != 0u, depth, P);
+  x.tcount = 0u;
+  x.ecount = 0u;
+  x.erad = 0.0f;
+  x.edist = 0.0f;
+  x.edir = mk3(0.0f, 0.0f, 0.0f);
+  for (uint32_t i = 0; i < c.d; ++i) {
+    TSample t = diffuse_sample(d, s, rng);
+    x.tdir[x.tcount] = t.dir;
+    x.tcoeff[x.tcount] = diffuse_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kDiffuse, t.prob, -1.0f);
+    x.tcount++;
+  }
+  for (uint32_t i = 0; i < c.sr; ++i) {
+    f3 nd = specular_dir(d, s);
+    float prob = spec_reflect_pdf(d, nd, s);
+    x.tdir[x.tcount] = nd;
+    x.tcoeff[x.tcount] = spec_reflect_dist(d, nd, s, m) / balance_den(S, d, nd, s, m, c, kSpecR, prob, -1.0f);
+    x.tcount++;
+  }
+  for (uint32_t i = 0; i < c.rr; ++i) {
+    TSample t = rough_reflect_sample(d, s, m, rng);
+    x.tdir[x.tcount] = t.dir;
+    x.tcoeff[x.tcount] = rough_reflect_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kRoughR, t.prob, -1.0f);
+    x.tcount++;
+  }
+  for (uint32_t i = 0; i < c.st; ++i) {
+    f3 nd = spec_trans_dir(d, s, m);
+    float prob = spec_trans_pdf(d, nd, s, m);
+    x.tdir[x.tcount] = nd;
+    x.tcoeff[x.tcount] = spec_trans_dist(d, nd, s, m) / balance_den(S, d, nd, s, m, c, kSpecT, prob, -1.0f);
+    x.tcount++;
+  }
+  for (uint32_t i = 0; i < c.rt; ++i) {
+    TSample t = rough_trans_sample(d, s, m, rng);
+    x.tdir[x.tcount] = t.dir;
+    x.tcoeff[x.tcount] = rough_trans_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kRoughT, t.prob, -1.0f);
+    x.tcount++;
+  }
+  for (uint32_t i = 0; i < c.l; ++i) {
+    ESample e = emission_sample(S, s, b, rng);
+    if (e.prob <= 1e-6f) continue;
+    float proj = CG_FABS(dot(s.sn, e.dir));
+    float coeff = (diffuse_dist(d, e.dir, s, m) + rough_reflect_dist(d, e.dir, s, m)) + rough_trans_dist(d, e.dir, s, m);
+    x.edir = e.dir;
+    x.erad = e.radiance * coeff * proj / balance_den(S, d, e.dir, s, m, c, kNone, -1.0f, e.prob);
+    x.edist = e.dist;
+    x.ecount++;
+  }
+  x.emission = m.emission;
+  x.pos = s.pos;
+  x.gn = s.gn;
+  x.acc = 0.0f;
+  x.iter = 0u;
+}
+// shading/spawn_surface_ray.wesl:6-15
+CG_HD f3 spawn_origin(f3 pos, f3 gn, f3 next) {
+  f3 off = dot(next, gn) > 0.0f ? gn : -gn;
+  return pos + off * 1e-3f;
+}
+// environment.wesl:7-25
+CG_HD float env_radiance(const DevScene& S, f3 dir, const SmitsBin& b) {
+  f3 d = normalize(dir);
+  float u = atan2f(d.y, d.x) / (2.0f * kPi) + 0.5f;
+  float v = acosf(clampf(d.z, -1.0f, 1.0f)) / kPi;
+  float wu = u - floorf(u);
+  float cv = clampf(1.0f - v, 0.0f, 0.999999f);
+  uint32_t x = umin(S.env_w - 1u, f2u(wu * static_cast<float>(S.env_w)));
+  uint32_t y = umin(S.env_h - 1u, f2u(cv * static_cast<float>(S.env_h)));
+  float4 t = S.env[y * S.env_w + x];
+  return rgb_radiance(mk3(t.x, t.y, t.z), b);
+}
+CG_HD uint32_t hmix(uint32_t h, uint32_t w) { return (h ^ w) * 16777619u; }
+
+// physics/srgb.wesl:5-27
+CG_HD float encode_channel(float lin) {
+  float v = fmx(lin, 0.0f);
+  float e = v <= 0.0031308f ? 12.92f * v : 1.055f * powf(v, 1.0f / 2.4f) - 0.055f;
+  return clampf(e, 0.0f, 1.0f);
+}
+CG_HD uint32_t xyz_to_rgba8(f3 xyz) {
+  f3 c0 = mk3(2.36461385f, -0.51516621f, 0.0052037f);
+  f3 c1 = mk3(-0.89654057f, 1.4264081f, -0.01440816f);
+  f3 c2 = mk3(-0.46807328f, 0.0887581f, 1.00920446f);
+  f3 lin = (c0 * xyz.x + c1 * xyz.y) + c2 * xyz.z;
+  uint32_t r = static_cast<uint32_t>(floorf(encode_channel(lin.x) * 255.0f + 0.5f));
+  uint32_t g = static_cast<uint32_t>(floorf(encode_channel(lin.y) * 255.0f + 0.5f));
+  uint32_t bb = static_cast<uint32_t>(floorf(encode_channel(lin.z) * 255.0f + 0.5f));
+  return r | (g << 8) | (bb << 16) | (255u << 24);
+}
+
+// compute.wesl:64-79
+CG_HD void camera_ray(const float* cpos, const float* cdir, const float* vp, uint32_t W, uint32_t H, uint32_t px,
+                      uint32_t py, f3& o, f3& d) {
+  float cx = (static_cast<float>(px) / static_cast<float>(W)) * 2.0f - 1.0f;
+  float cy = (static_cast<float>(py) / static_cast<float>(H)) * 2.0f - 1.0f;
+  f3 cd = ld3(cdir);
+  f3 dx = normalize(cross(cd, mk3(0.0f, 0.0f, 1.0f))) * vp[0];
+  f3 dy = normalize(cross(cd, dx)) * vp[1];
+  o = ld3(cpos);
+  d = normalize(cd + cx * dx + cy * dy);
+}
+
+}  // namespace cg
+
+#endif  // CG_CORE_H_

@@ -1,0 +1,108 @@
+// cg_layout.h — MI355X device layouts of the scene, built from the reference
+// SceneData (cg_scene.h) at upload time.
+//
+// Every record is a whole number of 16-byte vectors and either 64 B or a
+// multiple of 16 B, so one traversal step is a handful of dwordx4 loads from
+// one or two 128-byte lines:
+//   PairNode  64 B  both child boxes of one interior BVH node + child refs
+//   TriRec    48 B  v0, e1 = v1 - v0, e2 = v2 - v0, source triangle id
+//   InstRec   80 B  full column-major inverse transform, BLAS root ref, material
+//   VAttr     32 B  normal + both uv sets (read once, for the final hit)
+//   MatRec    64 B  the wavelength-independent part of a decoded material
+//   EmitRec   64 B  emissive triangle, its unit normal, RGB and area pdf
+#ifndef CG_LAYOUT_H_
+#define CG_LAYOUT_H_
+
+#include <stdint.h>
+
+// Child / root references (32 bits):
+//   00 | index   interior pair node
+//   01 | index   TLAS leaf: instance index
+//   1c | slot    BLAS leaf: first triangle slot, c = (triangle count - 1)
+//   kRefNone     empty (BLAS with no triangles, or empty TLAS)
+//   kRefMarker   traversal-stack marker: return from a BLAS to the TLAS
+#define CG_REF_KIND_MASK 0xC0000000u
+#define CG_REF_INDEX_MASK 0x3FFFFFFFu
+#define CG_REF_INST 0x40000000u
+#define CG_REF_LEAF 0x80000000u
+#define CG_REF_LEAF2 0x40000000u
+#define CG_REF_NONE 0x3FFFFFFFu
+#define CG_REF_MARKER 0x7FFFFFFFu
+
+struct __attribute__((aligned(16))) PairNode {
+  float lo_l[3];   // left child box
+  float hi_l[3];
+  float lo_r[3];   // right child box
+  float hi_r[3];
+  uint32_t ref_l;
+  uint32_t ref_r;
+  uint32_t _pad[2];
+};
+
+struct __attribute__((aligned(16))) TriRec {
+  float v0[3];
+  float e1[3];
+  float e2[3];
+  uint32_t tri_id;
+  uint32_t _pad[2];
+};
+
+struct __attribute__((aligned(16))) InstRec {
+  float inv[16];    // column-major
+  uint32_t root;    // BLAS root ref
+  uint32_t material;
+  uint32_t _pad[2];
+};
+
+struct __attribute__((aligned(16))) VAttr {
+  float nrm[3];
+  float uv0[2];
+  float uv1[2];
+  float _pad;
+};
+
+struct __attribute__((aligned(16))) MatRec {
+  float base[3];
+  float metallic;      // clamp(metallic_factor, 0, 1)
+  float roughness;     // clamp(roughness_factor, 0, 1)
+  float transmission;  // clamp(transmission_factor, 0, 1)
+  float ior;
+  float dispersion;
+  float emission[3];   // max(emissive_factor * emissive_strength, 0)
+  uint32_t volume;     // thickness_factor > 0
+  float alpha;         // roughness * roughness
+  float _pad[3];
+};
+
+struct __attribute__((aligned(16))) EmitRec {
+  float v0[3];
+  float ln_x;
+  float v1[3];
+  float ln_y;
+  float v2[3];
+  float ln_z;
+  float rgb[3];
+  float area_pdf;      // max(weight / area, 1e-3)
+};
+
+// Wavelength sampling table (one per library): the reference's 16-step
+// bisection (shader/compute/physics/wavelength.wesl:20-45) unrolled into a
+// heap of its 65535 midpoint CDF values and the 65536 possible outcomes.
+#define CG_WL_LEVELS 16
+#define CG_WL_LEAVES 65536
+struct __attribute__((aligned(16))) WlLeaf {
+  float lambda;
+  float prob;      // max(xyzMatchSum(lambda) / total, 1e-6)
+  float x, y, z;   // xyzMatch(lambda)
+  float _pad[3];
+};
+
+static_assert(sizeof(PairNode) == 64, "PairNode");
+static_assert(sizeof(TriRec) == 48, "TriRec");
+static_assert(sizeof(InstRec) == 80, "InstRec");
+static_assert(sizeof(VAttr) == 32, "VAttr");
+static_assert(sizeof(MatRec) == 64, "MatRec");
+static_assert(sizeof(EmitRec) == 64, "EmitRec");
+static_assert(sizeof(WlLeaf) == 32, "WlLeaf");
+
+#endif  // CG_LAYOUT_H_

@@ -1,0 +1,140 @@
+// cg_render.h — the compute pass for one pixel (compute.wesl:32-62,
+// render.wesl:7-18, trace.wesl:22-113), shared by the HIP kernel.
+//
+// The pixel consumes its RNG stream in exactly the reference's order: spp
+// samples, each a depth-first walk of the transport tree.  The open frames of
+// the walk live in F (one per level above the deepest, trace_depth - 1), the
+// BVH traversal stack behind `stk`.
+#ifndef CG_RENDER_H_
+#define CG_RENDER_H_
+
+#include "cg_core.h"
+
+namespace cg {
+
+struct ViewArgs {
+  DevScene S;
+  TraceParams P;
+  uint32_t W, H, spp, iter;
+  float cam_pos[3], cam_dir[3], viewport[2];
+  const float* wl_cdf;
+  const WlLeaf* wl_leaf;
+  float wl_avg;
+  cg_history_pixel* history;
+  uint32_t* rgba;
+  uint32_t* path_hash;
+  uint32_t* ray_count;
+  unsigned long long* stats;
+};
+
+
+template <int MF>
+CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, uint32_t* stk, int stride, TravStats& st,
+                        uint64_t& nrays, uint64_t& nshadow) {
+  const DevScene& S = A.S;
+  f3 co, cd;
+  camera_ray(A.cam_pos, A.cam_dir, A.viewport, A.W, A.H, px, py, co, cd);
+  const uint32_t hidx = px + py * A.W;
+  const float w = 1.0f / static_cast<float>(A.iter + 1u);
+  f3 lum = mk3(0.0f, 0.0f, 0.0f);
+  uint32_t rng = hidx;
+  if (A.iter > 0u) {
+    cg_history_pixel hp = A.history[hidx];
+    lum = mk3(hp.xyz[0], hp.xyz[1], hp.xyz[2]) * (1.0f - w);
+    rng = hp.rand_seed;
+  }
+  rng_step(rng);
+  uint32_t hash = 2166136261u;
+  Frame5 F[MF];
+  f3 acc = mk3(0.0f, 0.0f, 0.0f);
+  const float spp_f = static_cast<float>(A.spp);
+  for (uint32_t s = 0; s < A.spp; ++s) {
+    // sampleWavelength: walk the bisection heap.
+    float target = clampf(rng_float(rng), 0.0f, 0.999999f);
+    uint32_t node = 1u;
+#pragma unroll
+    for (int k = 0; k < CG_WL_LEVELS; ++k) node = 2u * node + (A.wl_cdf[node] < target ? 1u : 0u);
+    const WlLeaf L = A.wl_leaf[node - CG_WL_LEAVES];
+    const SmitsBin sb = smits_bin(L.lambda);
+    // trace(): depth-first over the transport tree.
+    int depth = 0;
+    f3 ro = co, rd = cd;
+    float ret = 0.0f;
+    for (;;) {
+      HitRec h = traverse<false>(S, ro, rd, kMax, stk, stride, st);
+      ++nrays;
+      hash = hmix(hmix(hash, h.inst), h.inst != CG_MISS_ID ? S.tris[h.slot].tri_id : CG_MISS_ID);
+      bool descended = false;
+      if (h.inst == CG_MISS_ID) {
+        ret = env_radiance(S, rd, sb);
+      } else if (depth >= A.P.D - 1) {
+        ret = 0.0f;
+      } else {
+        Surface sf = surface_at(S, ro, rd, h);
+        Frame5& f = F[depth];
+        forward_pass(S, rd, L.lambda, sb, sf, depth, A.P, rng, f);
+        if (f.ecount != 0u) {  // solveEmission (trace.wesl:91-113)
+          f3 lp = sf.pos + f.edir * f.edist;
+          f3 so = spawn_origin(sf.pos, sf.gn, f.edir);
+          float expected = length(so - lp);
+          HitRec sh = traverse<true>(S, so, f.edir, fmn(expected - 1e-3f, kMax), stk, stride, st);
+          ++nshadow;
+          bool occ = sh.inst != CG_MISS_ID;
+          hash = hmix(hash, 0xA5000000u | (occ ? 1u : 0u));
+          if (occ) f.erad = 0.0f;
+        }
+        if (f.tcount != 0u) {
+          f.iter = 1u;
+          rd = f.tdir[0];
+          ro = spawn_origin(f.pos, f.gn, rd);
+          ++depth;
+          descended = true;
+        } else {
+          float rad = 0.0f;
+          if (f.ecount != 0u) rad += f.erad;
+          ret = f.emission + rad;
+        }
+      }
+      if (descended) continue;
+      // return `ret` up the tree (collect.wesl:4-17, shading.wesl:48-53)
+      bool next_sibling = false;
+      while (depth > 0) {
+        Frame5& p = F[depth - 1];
+        p.acc += p.tcoeff[p.iter - 1u] * ret;
+        if (p.iter < p.tcount) {
+          rd = p.tdir[p.iter];
+          ro = spawn_origin(p.pos, p.gn, rd);
+          p.iter++;
+          next_sibling = true;
+          break;
+        }
+        float rad = p.acc;
+        if (p.ecount != 0u) rad += p.erad;
+        ret = p.emission + rad;
+        --depth;
+      }
+      if (!next_sibling) break;
+    }
+    // illuminate (luminance.wesl:13-16), averaged over spp (render.wesl:13-16)
+    acc.x += L.x * ret / A.wl_avg / L.prob / spp_f;
+    acc.y += L.y * ret / A.wl_avg / L.prob / spp_f;
+    acc.z += L.z * ret / A.wl_avg / L.prob / spp_f;
+  }
+  lum = lum + acc * w;
+  cg_history_pixel out;
+  out.xyz[0] = lum.x;
+  out.xyz[1] = lum.y;
+  out.xyz[2] = lum.z;
+  rng_step(rng);
+  out.rand_seed = rng;
+  A.history[hidx] = out;
+  A.rgba[hidx] = xyz_to_rgba8(lum);
+  if (A.path_hash) {
+    A.path_hash[hidx] = hash;
+    A.ray_count[hidx] = static_cast<uint32_t>(nrays + nshadow);
+  }
+}
+
+}  // namespace cg
+
+#endif  // CG_RENDER_H_

@@ -1,0 +1,119 @@
+/*
+ * cg_device.h — the C-ABI between the crystal::graphics host layer and the
+ * MI355X (gfx950) HIP backend.  It replaces the reference's WebGPU backend
+ * (src/wgpu/) and its WGSL compute pass; plain pointers and sizes only.
+ *
+ * Conventions (mirroring the reference's std::expected<..., Error> returns,
+ * src/error_stack.h:12-28):
+ *   - every call returns CG_OK (0) or a CG_ERR_* code;
+ *   - on failure the message is kept per env (cg_env_last_error) and per
+ *     thread (cg_last_error) until the next failing call;
+ *   - device work is enqueued on the env's HIP stream and the call returns
+ *     without waiting (as View's submit does, src/wgpu/view.cpp:81-87), except
+ *     where a function says it synchronises;
+ *   - one env is not re-entrant: calls on one env come from one thread.
+ * Entry points that read or write "device" buffers take pointers obtained
+ * from cg_device_alloc (HBM); "host" buffers are ordinary memory.
+ */
+#ifndef CG_DEVICE_H_
+#define CG_DEVICE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "cg_scene.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CG_OK = 0,
+  CG_ERR_INVALID = 1,      /* bad argument or unsupported input */
+  CG_ERR_HIP = 2,          /* HIP runtime error */
+  CG_ERR_OUT_OF_MEMORY = 3,
+  CG_ERR_UNSUPPORTED = 4,  /* scene feature the kernels do not implement */
+  CG_ERR_STATE = 5         /* call out of order (e.g. view before upload) */
+};
+
+typedef struct cg_env cg_env;
+typedef struct cg_scene cg_scene;
+
+/* Traversal counters accumulated by cg_view / cg_trace_* while counting is
+ * enabled: closest-hit and any-hit queries, 64-B pair nodes fetched, 48-B
+ * triangle records tested, 80-B instance records entered. */
+typedef struct cg_stats {
+  uint64_t rays;
+  uint64_t shadow_rays;
+  uint64_t nodes;
+  uint64_t tris;
+  uint64_t insts;
+} cg_stats;
+
+/* Create the backend on HIP device `device` for `conf` (render size, spp,
+ * depth, lobe caps), allocating the history and target buffers.  Replaces
+ * wgpu::CreateEnv (src/wgpu/environment.cpp:19-108); trace_depth is clamped
+ * to [1, 64] as there (:20). */
+int cg_env_create(const cg_conf* conf, int device, cg_env** out);
+/* Release everything the env owns; replaces wgpu::Env::Term
+ * (src/wgpu/environment.cpp:126-138). */
+int cg_env_destroy(cg_env* env);
+/* Last error message of this env / of this thread ("" if none). */
+const char* cg_env_last_error(const cg_env* env);
+const char* cg_last_error(void);
+/* Wait for all enqueued work; replaces wgpu::Env::Poll
+ * (src/wgpu/environment.cpp:110-116). */
+int cg_env_sync(cg_env* env);
+/* The env's hipStream_t, for callers that time or order work on it. */
+void* cg_env_stream(cg_env* env);
+/* Enable (1) or disable (0) traversal counting and, for cg_view, per-pixel
+ * path records (hash of the hit-id sequence and query count). */
+int cg_env_set_counting(cg_env* env, int enable);
+int cg_env_set_path_records(cg_env* env, int enable);
+int cg_stats_read(cg_env* env, cg_stats* out);  /* synchronises */
+int cg_stats_reset(cg_env* env);
+
+/* Upload a flattened scene and build the device layout (pair nodes, triangle
+ * records, instance records, material and emitter records, wavelength and
+ * alias tables).  Replaces WriteScene + UpdateComputeBindGroup2
+ * (src/wgpu/resource.cpp:338-414, src/wgpu/compute.cpp:332-372).
+ * Materials with textures give CG_ERR_UNSUPPORTED. */
+int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out);
+int cg_scene_release(cg_env* env, cg_scene* scene);
+/* Device bytes held by a scene, and the traversal stack depth it needs. */
+int cg_scene_footprint(const cg_scene* scene, uint64_t* bytes, uint32_t* stack_depth);
+
+/* One progressive frame of the compute pass (shader/compute.wesl:32-62):
+ * camera ray per pixel, spp spectral samples of the depth-first transport
+ * tree in the reference's per-pixel RNG order, running-mean CIE XYZ history,
+ * sRGB rgba8 target.  iter_count == 0 starts a new accumulation.  Replaces
+ * wgpu::Env::View (src/wgpu/view.cpp:16-88). */
+int cg_view(cg_env* env, cg_scene* scene, const cg_camera* camera, uint32_t iter_count);
+/* Headless read-back of the target (new; the reference only presents):
+ * history (W*H cg_history_pixel) and/or rgba8 (W*H*4).  Synchronises. */
+int cg_read_target(cg_env* env, cg_history_pixel* history, uint8_t* rgba8);
+/* Per-pixel path records of the last cg_view (W*H each).  Synchronises. */
+int cg_read_path_records(cg_env* env, uint32_t* path_hash, uint32_t* ray_count);
+/* Overwrite the history buffer (W*H entries) from host memory. */
+int cg_write_history(cg_env* env, const cg_history_pixel* history);
+
+/* Batch closest hit, one query per ray: the reference's rayHit
+ * (shader/compute/structural/bvh.wesl:52-130) as a standalone entry point.
+ * rays / hits (and uv: 2 floats per ray, nullable) are device buffers. */
+int cg_trace_closest(cg_env* env, cg_scene* scene, const cg_ray* rays, uint32_t n, cg_hit* hits, float* uv);
+/* Batch occlusion: occluded[i] = 1 iff a hit exists with t < tmax[i]
+ * (trace.wesl:104-111 asks exactly this of its shadow rays). */
+int cg_trace_occluded(cg_env* env, cg_scene* scene, const cg_ray* rays, const float* tmax, uint32_t n,
+                      uint32_t* occluded);
+
+/* HBM buffers for the batch entry points. */
+int cg_device_alloc(cg_env* env, size_t bytes, void** dptr);
+int cg_device_free(cg_env* env, void* dptr);
+int cg_memcpy_h2d(cg_env* env, void* dst, const void* src, size_t bytes);  /* synchronises */
+int cg_memcpy_d2h(cg_env* env, void* dst, const void* src, size_t bytes);  /* synchronises */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CG_DEVICE_H_ */

@@ -1,0 +1,96 @@
+/*
+ * cg_oracle.h — ORACLE: test infrastructure only.
+ *
+ * A scalar CPU restatement of the reference's spectral path-tracing core:
+ * the WGSL compute pass (src/wgpu/shader/compute.wesl and the compute/ modules) and the
+ * host scene preparation (src/pathtracing/ headers).  It exists to check the
+ * product (the HIP library under crystalgraphics_amd/) and is only ever
+ * loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ * Nothing in the product links, loads or calls it.
+ *
+ * Float semantics are declared, not inherited from Tint/Dawn:
+ *   - every + - * / sqrt is one IEEE-754 binary32 operation, evaluated left to
+ *     right as written in the WGSL, with no FMA contraction;
+ *   - min/max/clamp ignore a NaN operand (IEEE minNum/maxNum, fminf/fmaxf);
+ *   - f32 -> u32 saturates and maps NaN / negatives to 0;
+ *   - runtime array indices are clamped to the array (WebGPU robustness);
+ *   - pow(x, 2) = x*x and pow(x, 4) = (x*x)*(x*x);
+ *   - exp/cos/sin/tan/acos/atan2/pow(x, 1/2.4) are the C library's.
+ */
+#ifndef CG_ORACLE_H_
+#define CG_ORACLE_H_
+
+#include <stdint.h>
+
+#include "../include/cg_scene.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_scene orc_scene;
+
+/* Scene preparation (src/pathtracing/scene_data.h:38-54): BLAS per primitive
+ * object, TLAS, instances, emissive buffer, alias table, fallback sky. */
+orc_scene* orc_scene_build(const cg_vertex* vertices, uint32_t vertex_count,
+                           const uint32_t* indices, uint32_t index_count,
+                           const cg_material* materials, uint32_t material_count,
+                           const cg_primitive* prims, uint32_t prim_count,
+                           const float* env_rgba32f, uint32_t env_w, uint32_t env_h);
+/* Take a SceneData built elsewhere (copied). */
+orc_scene* orc_scene_wrap(const cg_scene_data* sd);
+void orc_scene_free(orc_scene* s);
+/* Pointers into the oracle scene's storage. */
+void orc_scene_get(const orc_scene* s, cg_scene_data* out);
+
+/* Closest hit per ray (bvh.wesl:52-186).  second_t (nullable) receives the
+ * second-smallest hit distance among distinct triangles (1e6 if none), so a
+ * caller can exclude ties.  counts (nullable) receives per ray
+ * {TLAS nodes, BLAS nodes, triangles, instances} visited.  cull != 0 skips
+ * boxes entered beyond the current closest hit (same result, faster). */
+void orc_trace_closest(const orc_scene* s, const cg_ray* rays, uint32_t n,
+                       cg_hit* hits, float* uv, float* second_t,
+                       uint32_t* counts, int cull, int threads);
+
+/* The compute pass over pixel rows [y0, y1) (compute.wesl:32-62).  history is
+ * the full W*H buffer, read when iter_count > 0 and written for the rows.
+ * rgba8 (nullable, W*H*4), path_hash (nullable, W*H) receives a hash of the
+ * pixel's sequence of hit ids and shadow outcomes, ray_count (nullable, W*H)
+ * its closest-hit queries.  Returns 0, or -1 on an unsupported scene. */
+int orc_render(const orc_scene* s, const cg_conf* conf, const cg_camera* cam,
+               uint32_t iter_count, cg_history_pixel* history, uint8_t* rgba8,
+               uint32_t* path_hash, uint32_t* ray_count,
+               uint32_t y0, uint32_t y1, int threads, int cull);
+
+/* Unit-level pieces, for known-answer and closed-form checks. */
+uint32_t orc_rng_step(uint32_t state);
+float orc_rand_float(uint32_t* state);
+float orc_erf(float x);
+void orc_xyz_match(float lambda, float* xyz);
+float orc_xyz_match_sum_integral(float lo, float hi);
+void orc_sample_wavelength(float u, float* lambda, float* prob);
+void orc_illuminate(float lambda, float radiance, float* xyz);
+float orc_smits(int curve, float lambda); /* 0 white 1 red 2 green 3 blue 4 magenta 5 cyan 6 yellow */
+float orc_rgb_to_reflectance(const float* rgb, float lambda);
+float orc_rgb_to_radiance(const float* rgb, float lambda);
+float orc_dispersed_ior(float nd, float dispersion, float lambda);
+float orc_fresnel(const float* dir, const float* normal, float ior_re, float ior_im);
+void orc_alias_build(const double* weights, uint32_t n, cg_alias_entry* out);
+uint32_t orc_alias_pick(const cg_alias_entry* table, uint32_t n, float u1, float u2);
+void orc_srgb(const float* xyz, float* rgb);
+void orc_camera_ray(const cg_camera* cam, uint32_t w, uint32_t h,
+                    uint32_t px, uint32_t py, cg_ray* out);
+int orc_intersect_triangle(const cg_ray* ray, const float* v0, const float* v1,
+                           const float* v2, float* tuv);
+int orc_intersect_aabb(const cg_ray* ray, const float* lb, const float* ub);
+/* Sampling of one lobe at a surface for unit tests: model 1 diffuse,
+ * 3 rough reflect, 5 rough transmit.  Returns the pdf, writes the direction. */
+float orc_sample_lobe(int model, const float* curr_dir, const float* shading_norm,
+                      float alpha, float ior_re, float ior_im, float transmission,
+                      uint32_t* rng_state, float* next_dir);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CG_ORACLE_H_ */
