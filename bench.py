@@ -1,0 +1,188 @@
+"""Benchmark: Mrays/s of the spectral path tracer on one MI355X.
+
+Workload (one step = one frame of the compute pass): 1280x720, 16 spp,
+trace depth 8, 16 instances of a 131,072-triangle noise-displaced sphere
+(2.1M triangles) with random rigid transforms and scale U(0.5, 2), 4096
+emissive triangles in the alias table, materials drawn from
+{diffuse, rough metal, dispersive glass}.  The scene is built and uploaded
+before timing; the timed region is K frames of cg_view on the library's HIP
+stream, bracketed by device synchronisation and measured with HIP events on
+that stream.  A ray is one closest-hit or occlusion query (camera, transport
+and shadow rays); the count comes from the kernel's own counters on an
+untimed frame with identical work.
+
+Prints one JSON line.  --gpus N > 1 is rejected: this path runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes of one traversal query in the device layout: ray in (32)
+# + hit out (16) + 64 B per pair node fetched + 48 B per triangle record
+# tested + 80 B per instance record entered.
+RAY_IO_BYTES = 32 + 16
+PAIR_BYTES, TRI_BYTES, INST_BYTES = 64, 48, 80
+
+
+class HipEvents:
+    """hipEvent timing on a given stream (the library's, not torch's)."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        for name in ("hipEventCreate", "hipEventRecord", "hipEventSynchronize", "hipEventElapsedTime",
+                     "hipEventDestroy"):
+            getattr(self.hip, name).restype = C.c_int
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def create(self):
+        e = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, C.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return float(ms.value)
+
+
+def cpu_baseline(scene, sd, conf, cam, budget_s: float, threads: int) -> dict:
+    """The oracle (a scalar C++ restatement of the reference's compute pass)
+    on host cores over a bounded band of rows of the same frame."""
+    from oracle.binding import OracleScene
+
+    orc = OracleScene.wrap(sd.c)
+    rows_done, rays, elapsed = 0, 0, 0.0
+    H = conf.render_height
+    band = max(1, threads // 4)
+    y = H // 2 - 8
+    while elapsed < budget_s and rows_done < H:
+        y0 = (y + rows_done) % H
+        y1 = min(H, y0 + band)
+        t0 = time.perf_counter()
+        out = orc.render(conf, cam, rows=(y0, y1), threads=threads, cull=True)
+        elapsed += time.perf_counter() - t0
+        rays += int(out["ray_count"][y0:y1].sum())
+        rows_done += y1 - y0
+    return {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{rows_done} rows x {conf.render_width} px of the same frame ({rays} queries, "
+                      f"{elapsed:.1f} s), oracle with ordered culling traversal (same hits)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark measures one GPU; --gpus must be 1 (the path is single-GPU: "
+                                   "pixels, samples and jobs shard across GPUs without mid-frame exchange)"}))
+        return 2
+
+    import crystalgraphics_amd as cg
+
+    t0 = time.perf_counter()
+    scene = cg.HostScene.generate("c1", seed=args.seed)
+    desc = scene.describe()
+    sd = scene.scene_data(share_blas=True, threads=16)
+    t_prep = time.perf_counter() - t0
+    conf = desc["conf"]
+    cam = desc["camera"]
+    W, H = conf.render_width, conf.render_height
+
+    env = cg.DeviceEnv(conf)
+    ds = env.upload(sd)
+    stream = env.stream
+    ev = HipEvents()
+
+    # Untimed counted frame: rays and node/triangle/instance fetches per frame.
+    env.set_counting(True)
+    env.reset_stats()
+    env.view(ds, cam, 0)
+    env.sync()
+    st = env.stats()
+    env.set_counting(False)
+    rays = st["rays"] + st["shadow_rays"]
+
+    for _ in range(args.warmup):
+        env.view(ds, cam, 0)
+    env.sync()
+    e0, e1 = ev.create(), ev.create()
+    ev.record(e0, stream)
+    for _ in range(args.steps):
+        env.view(ds, cam, 0)
+    ev.record(e1, stream)
+    env.sync()
+    ms_total = ev.elapsed_ms(e0, e1)
+    ms = ms_total / args.steps
+    mrays = rays / (ms * 1e-3) / 1e6
+
+    alg_bytes = RAY_IO_BYTES * rays + PAIR_BYTES * st["nodes"] + TRI_BYTES * st["tris"] + INST_BYTES * st["insts"]
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+
+    base = None
+    if not args.no_cpu:
+        base = cpu_baseline(scene, sd, conf, cam, args.cpu_seconds, min(16, os.cpu_count() or 1))
+
+    out = {
+        "metric": "Mrays/s at matched rMSE vs CPU ref; p50/p99 job latency in serve loop",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {
+            "workload": "single-frame plain path: 1280x720, 16 spp, depth 8, 16 instances of a 131072-tri "
+                        "noise sphere (2.1M tris), 4096 emissive tris, diffuse/rough-metal/dispersive-glass",
+            "render": [W, H], "spp": conf.sample_count, "trace_depth": conf.trace_depth,
+            "rays_per_frame": rays, "shadow_rays_per_frame": st["shadow_rays"],
+            "rng_order": "reference (per-pixel depth-first stream)",
+            "scene_prep_s": round(t_prep, 2),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "k_view",
+            "per_launch": {"pair_nodes": st["nodes"], "triangles": st["tris"], "instances": st["insts"],
+                           "queries": rays, "algorithmic_bytes": alg_bytes},
+        },
+        "cpu_baseline": base,
+        "latency_ms": None,
+    }
+    print(json.dumps(out))
+    env.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

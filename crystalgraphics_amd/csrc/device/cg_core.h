@@ -6,8 +6,9 @@
 // NaN-ignoring fminf/fmaxf, f32->u32 conversions saturate, and runtime array
 // indices are clamped the way WebGPU robustness clamps them.  That makes
 // traversal, intersection, RNG, wavelength sampling, alias sampling, Smits
-// lifts and Fresnel terms bit-identical to a CPU restatement; only
-// cos/sin/tan/acos/atan2/exp/pow may differ in the last ulp.
+// lifts, Fresnel, GGX terms and lobe sampling bit-identical to a CPU
+// restatement; only acos/atan2 (the sky texel, a discrete choice) and pow
+// (the sRGB target) may differ in the last ulp.
 #ifndef CG_CORE_H_
 #define CG_CORE_H_
 
@@ -453,6 +454,33 @@ CG_HD Frame tangent_space(f3 n) {
 CG_HD f3 to_world(const Frame& f, f3 v) { return (f.t * v.x + f.b * v.y) + f.n * v.z; }
 CG_HD f3 to_local(const Frame& f, f3 v) { return mk3(dot(f.t, v), dot(f.b, v), dot(f.n, v)); }
 
+// sin and cos of a sampling angle in [0, 2 pi], as declared for this path:
+// Cody-Waite reduction by pi/2 (three-part constant) and Taylor polynomials of
+// degree 9 (sin) and 10 (cos) on [-pi/4, pi/4], evaluated in this order.
+// Accuracy is about one ulp, well inside WGSL's 2^-11 bound for sin/cos.
+CG_HD void sincos_decl(float x, float& s, float& c) {
+  const float fk = floorf(x * 0x1.45f306p-1f + 0.5f);
+  const float r = ((x - fk * 0x1.92p+0f) - fk * 0x1.fb54p-12f) - fk * 0x1.10b46p-30f;
+  const float r2 = r * r;
+  float ps = 0x1.71de3ap-19f;
+  ps = -0x1.a01a02p-13f + r2 * ps;
+  ps = 0x1.111112p-7f + r2 * ps;
+  ps = -0x1.555556p-3f + r2 * ps;
+  const float sr = r + (r * r2) * ps;
+  float pc = -0x1.27e4fcp-22f;
+  pc = 0x1.a01a02p-16f + r2 * pc;
+  pc = -0x1.6c16c2p-10f + r2 * pc;
+  pc = 0x1.555556p-5f + r2 * pc;
+  pc = -0.5f + r2 * pc;
+  const float cr = 1.0f + r2 * pc;
+  switch (static_cast<int>(fk) & 3) {
+    case 0: s = sr; c = cr; break;
+    case 1: s = cr; c = -sr; break;
+    case 2: s = -sr; c = -cr; break;
+    default: s = -cr; c = sr; break;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Lobes (shading/pdf/*.wesl, shading/dist/*.wesl).
 struct TSample {
@@ -472,7 +500,9 @@ CG_HD TSample diffuse_sample(f3 d, const Surface& s, uint32_t& rng) {
   f3 macro = oriented(d, s.sn);
   float r = CG_SQRT(rng_float(rng));
   float angle = 2.0f * kPi * rng_float(rng);
-  f3 local = mk3(r * cosf(angle), r * sinf(angle), CG_SQRT(fmx(0.0f, 1.0f - r * r)));
+  float sa, ca;
+  sincos_decl(angle, sa, ca);
+  f3 local = mk3(r * ca, r * sa, CG_SQRT(fmx(0.0f, 1.0f - r * r)));
   f3 next = to_world(tangent_space(macro), local);
   return TSample{next, fmx(dot(next, macro), 0.0f) / kPi};
 }
@@ -512,14 +542,16 @@ CG_HD float ggx_d(f3 micro, f3 macro, float alpha) {
   float den = kPi * (c2 * c2) * (at * at);
   return alpha / fmx(den, 1e-6f);
 }
-CG_HD float ggx_g1(float angle, float alpha) {
-  float t = tanf(angle);
-  return 1.0f / (1.0f + (CG_SQRT(1.0f + alpha * t * t) - 1.0f) / 2.0f);
+// tan(acos(c))^2 of a projection c in [1e-6, 1], as its exact identity
+// max(0, 1 - c^2) / c^2 (the composed form is ill-conditioned at grazing c).
+CG_HD float tan2_acos(float c) { return fmx(0.0f, 1.0f - c * c) / (c * c); }
+CG_HD float ggx_g1(float c, float alpha) {
+  float t2 = tan2_acos(c);
+  return 1.0f / (1.0f + (CG_SQRT(1.0f + alpha * t2) - 1.0f) / 2.0f);
 }
-CG_HD float ggx_g2(float a1, float a2, float alpha) {
-  float t1 = tanf(a1), t2 = tanf(a2);
-  return 1.0f / (1.0f + (CG_SQRT(1.0f + alpha * t1 * t1) - 1.0f) / 2.0f +
-                 (CG_SQRT(1.0f + alpha * t2 * t2) - 1.0f) / 2.0f);
+CG_HD float ggx_g2(float c1, float c2, float alpha) {
+  float t1 = tan2_acos(c1), t2 = tan2_acos(c2);
+  return 1.0f / (1.0f + (CG_SQRT(1.0f + alpha * t1) - 1.0f) / 2.0f + (CG_SQRT(1.0f + alpha * t2) - 1.0f) / 2.0f);
 }
 CG_HD f3 sample_micro_normal(f3 d, const Surface& s, float alpha, uint32_t& rng) {
   f3 macro = oriented(d, s.sn);
@@ -531,8 +563,10 @@ CG_HD f3 sample_micro_normal(f3 d, const Surface& s, float alpha, uint32_t& rng)
   f3 ty = cross(st, tx);
   float radius = CG_SQRT(rng_float(rng));
   float angle = 2.0f * kPi * rng_float(rng);
-  float px = radius * cosf(angle);
-  float py = radius * sinf(angle);
+  float sa, ca;
+  sincos_decl(angle, sa, ca);
+  float px = radius * ca;
+  float py = radius * sa;
   float vis = 0.5f * (1.0f + st.z);
   py = mixf(CG_SQRT(fmx(0.0f, 1.0f - px * px)), py, vis);
   f3 lsm = px * tx + py * ty + CG_SQRT(fmx(0.0f, 1.0f - px * px - py * py)) * st;
@@ -542,7 +576,7 @@ CG_HD f3 sample_micro_normal(f3 d, const Surface& s, float alpha, uint32_t& rng)
 CG_HD float reflect_pdf(f3 d, f3 micro, f3 macro, float alpha) {
   float cmp = fmx(dot(-d, macro), 1e-6f);
   float cmi = fmx(dot(-d, micro), 1e-6f);
-  float vnp = ggx_d(micro, macro, alpha) * ggx_g1(acosf(cmp), alpha) * cmi / cmp;
+  float vnp = ggx_d(micro, macro, alpha) * ggx_g1(cmp, alpha) * cmi / cmp;
   return vnp / (4.0f * cmi);
 }
 CG_HD float rough_reflect_pdf(f3 d, f3 next, const Surface& s, const Mat& m) {
@@ -570,14 +604,14 @@ CG_HD float rough_reflect_dist(f3 d, f3 next, const Surface& s, const Mat& m) {
   IOR rel = rel_ior(raw, d, m);
   float cmp = fmx(dot(-d, macro), 1e-6f);
   float nmp = fmx(dot(next, macro), 1e-6f);
-  float rd = ggx_d(micro, macro, m.alpha) * ggx_g2(acosf(cmp), acosf(nmp), m.alpha) / (4.0f * cmp * nmp);
+  float rd = ggx_d(micro, macro, m.alpha) * ggx_g2(cmp, nmp, m.alpha) / (4.0f * cmp * nmp);
   return fresnel(d, micro, rel) * rd * np;
 }
 CG_HD float rel_value(IOR i) { return fmx(i.re, 1e-6f); }
 CG_HD float transmit_pdf(f3 d, f3 micro, f3 macro, f3 next, float tior, float alpha) {
   float cmp = fmx(dot(-d, macro), 1e-6f);
   float cmi = fmx(dot(-d, micro), 1e-6f);
-  float vnp = ggx_d(micro, macro, alpha) * ggx_g1(acosf(cmp), alpha) * cmi / cmp;
+  float vnp = ggx_d(micro, macro, alpha) * ggx_g1(cmp, alpha) * cmi / cmp;
   float nmi = dot(next, micro);
   float den = nmi + cmi / tior;
   return vnp * cmi / fmx(den * den, 1e-6f);
@@ -616,7 +650,7 @@ CG_HD float rough_trans_dist(f3 d, f3 next, const Surface& s, const Mat& m) {
   float cmi = fmx(dot(-d, micro), 1e-6f);
   float nmi = dot(next, micro);
   float den = nmi + cmi / tior;
-  float td = m.transmission * ggx_d(micro, macro, m.alpha) * ggx_g2(acosf(cmp), acosf(nmp), m.alpha) *
+  float td = m.transmission * ggx_d(micro, macro, m.alpha) * ggx_g2(cmp, nmp, m.alpha) *
              CG_FABS(cmi * nmi) / fmx(cmp * nmp * den * den, 1e-6f);
   return (1.0f - fresnel(d, micro, rel)) * td * np;
 }
@@ -813,7 +847,7 @@ CG_HD f3 spawn_origin(f3 pos, f3 gn, f3 next) {
   return pos + off * 1e-3f;
 }
 // environment.wesl:7-25
-CG_HD float env_radiance(const DevScene& S, f3 dir, const SmitsBin& b) {
+CG_HD float env_radiance(const DevScene& S, f3 dir, const SmitsBin& b, uint32_t& texel) {
   f3 d = normalize(dir);
   float u = atan2f(d.y, d.x) / (2.0f * kPi) + 0.5f;
   float v = acosf(clampf(d.z, -1.0f, 1.0f)) / kPi;
@@ -821,7 +855,8 @@ CG_HD float env_radiance(const DevScene& S, f3 dir, const SmitsBin& b) {
   float cv = clampf(1.0f - v, 0.0f, 0.999999f);
   uint32_t x = umin(S.env_w - 1u, f2u(wu * static_cast<float>(S.env_w)));
   uint32_t y = umin(S.env_h - 1u, f2u(cv * static_cast<float>(S.env_h)));
-  float4 t = S.env[y * S.env_w + x];
+  texel = y * S.env_w + x;
+  float4 t = S.env[texel];
   return rgb_radiance(mk3(t.x, t.y, t.z), b);
 }
 CG_HD uint32_t hmix(uint32_t h, uint32_t w) { return (h ^ w) * 16777619u; }

@@ -66,7 +66,9 @@ CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, uint32_t* s
       hash = hmix(hmix(hash, h.inst), h.inst != CG_MISS_ID ? S.tris[h.slot].tri_id : CG_MISS_ID);
       bool descended = false;
       if (h.inst == CG_MISS_ID) {
-        ret = env_radiance(S, rd, sb);
+        uint32_t texel = 0u;
+        ret = env_radiance(S, rd, sb, texel);
+        hash = hmix(hash, 0x5E000000u ^ texel);
       } else if (depth >= A.P.D - 1) {
         ret = 0.0f;
       } else {

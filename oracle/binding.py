@@ -37,6 +37,7 @@ def lib() -> C.CDLL:
             ("orc_trace_closest", None, [P, P, U32, P, P, P, P, C.c_int, C.c_int]),
             ("orc_render", C.c_int, [P, C.POINTER(Conf), C.POINTER(Camera), U32, P, P, P, P, U32, U32, C.c_int,
                                      C.c_int]),
+            ("orc_sincos", None, [F, FP, FP]),
             ("orc_rng_step", U32, [U32]),
             ("orc_rand_float", F, [C.POINTER(U32)]),
             ("orc_erf", F, [F]),

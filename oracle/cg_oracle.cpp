@@ -358,6 +358,33 @@ struct Surface {
   f2 uv0, uv1;
 };
 
+// sin and cos of a sampling angle in [0, 2 pi], as declared for this path:
+// Cody-Waite reduction by pi/2 (three-part constant) and Taylor polynomials of
+// degree 9 (sin) and 10 (cos) on [-pi/4, pi/4], evaluated in this order.
+// Accuracy is about one ulp, well inside WGSL's 2^-11 bound for sin/cos.
+void sincos_decl(float x, float& s, float& c) {
+  const float fk = std::floor(x * 0x1.45f306p-1f + 0.5f);
+  const float r = ((x - fk * 0x1.92p+0f) - fk * 0x1.fb54p-12f) - fk * 0x1.10b46p-30f;
+  const float r2 = r * r;
+  float ps = 0x1.71de3ap-19f;
+  ps = -0x1.a01a02p-13f + r2 * ps;
+  ps = 0x1.111112p-7f + r2 * ps;
+  ps = -0x1.555556p-3f + r2 * ps;
+  const float sr = r + (r * r2) * ps;
+  float pc = -0x1.27e4fcp-22f;
+  pc = 0x1.a01a02p-16f + r2 * pc;
+  pc = -0x1.6c16c2p-10f + r2 * pc;
+  pc = 0x1.555556p-5f + r2 * pc;
+  pc = -0.5f + r2 * pc;
+  const float cr = 1.0f + r2 * pc;
+  switch (static_cast<int>(fk) & 3) {
+    case 0: s = sr; c = cr; break;
+    case 1: s = cr; c = -sr; break;
+    case 2: s = -sr; c = -cr; break;
+    default: s = -cr; c = sr; break;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Lobes.  pdf/*.wesl and dist/*.wesl.
 struct TSample {
@@ -370,7 +397,9 @@ TSample diffuse_sample(v3 d, const Surface& s, uint32_t& rng) {
   v3 macro = oriented(d, s.sn);
   float r = std::sqrt(rand_float(rng));
   float angle = 2.0f * PI * rand_float(rng);
-  v3 local = mk(r * std::cos(angle), r * std::sin(angle), std::sqrt(fmx(0.0f, 1.0f - r * r)));
+  float sa, ca;
+  sincos_decl(angle, sa, ca);
+  v3 local = mk(r * ca, r * sa, std::sqrt(fmx(0.0f, 1.0f - r * r)));
   v3 next = to_world(tangent_space(macro), local);
   return TSample{next, fmx(dot(next, macro), 0.0f) / PI};
 }
@@ -433,14 +462,20 @@ float ggx_d(v3 micro, v3 macro, float alpha) {
   float den = PI * (c2 * c2) * (at * at);
   return alpha / fmx(den, 1e-6f);
 }
-float ggx_g1(float angle, float alpha) {
-  float t = std::tan(angle);
-  return 1.0f / (1.0f + (std::sqrt(1.0f + alpha * t * t) - 1.0f) / 2.0f);
+// The reference writes tan(acos(c)) * tan(acos(c)) for a projection
+// c in [1e-6, 1] (rough_reflection.wesl:114, 122-123, 185-195).  That
+// composition is ill-conditioned at grazing c (one ulp of acos near pi/2 moves
+// tan by ~1e-7 / c), and WGSL leaves acos's accuracy to the implementation, so
+// its value is declared here as the exact identity max(0, 1 - c^2) / c^2.
+inline float tan2_acos(float c) { return fmx(0.0f, 1.0f - c * c) / (c * c); }
+float ggx_g1(float c, float alpha) {
+  float t2 = tan2_acos(c);
+  return 1.0f / (1.0f + (std::sqrt(1.0f + alpha * t2) - 1.0f) / 2.0f);
 }
-float ggx_g2(float a1, float a2, float alpha) {
-  float t1 = std::tan(a1), t2 = std::tan(a2);
-  return 1.0f / (1.0f + (std::sqrt(1.0f + alpha * t1 * t1) - 1.0f) / 2.0f +
-                 (std::sqrt(1.0f + alpha * t2 * t2) - 1.0f) / 2.0f);
+float ggx_g2(float c1, float c2, float alpha) {
+  float t1 = tan2_acos(c1), t2 = tan2_acos(c2);
+  return 1.0f / (1.0f + (std::sqrt(1.0f + alpha * t1) - 1.0f) / 2.0f +
+                 (std::sqrt(1.0f + alpha * t2) - 1.0f) / 2.0f);
 }
 // pdf/rough_reflection.wesl:52-103 (identical in rough_transmission.wesl:68-119)
 v3 sample_micro_normal(v3 d, const Surface& s, float alpha, uint32_t& rng) {
@@ -454,8 +489,10 @@ v3 sample_micro_normal(v3 d, const Surface& s, float alpha, uint32_t& rng) {
   v3 ty = cross(st, tx);
   float radius = std::sqrt(rand_float(rng));
   float angle = 2.0f * PI * rand_float(rng);
-  float px = radius * std::cos(angle);
-  float py = radius * std::sin(angle);
+  float sa, ca;
+  sincos_decl(angle, sa, ca);
+  float px = radius * ca;
+  float py = radius * sa;
   float vis = 0.5f * (1.0f + st.z);
   py = mixf(std::sqrt(fmx(0.0f, 1.0f - px * px)), py, vis);
   v3 lsm = px * tx + py * ty + std::sqrt(fmx(0.0f, 1.0f - px * px - py * py)) * st;
@@ -465,7 +502,7 @@ v3 sample_micro_normal(v3 d, const Surface& s, float alpha, uint32_t& rng) {
 float reflect_pdf(v3 d, v3 micro, v3 macro, float alpha) {  // rough_reflection.wesl:105-118
   float cmp = fmx(dot(-d, macro), 1e-6f);
   float cmi = fmx(dot(-d, micro), 1e-6f);
-  float vnp = ggx_d(micro, macro, alpha) * ggx_g1(std::acos(cmp), alpha) * cmi / cmp;
+  float vnp = ggx_d(micro, macro, alpha) * ggx_g1(cmp, alpha) * cmi / cmp;
   return vnp / (4.0f * cmi);
 }
 TSample rough_reflect_sample(v3 d, const Surface& s, const Mat& m, uint32_t& rng) {
@@ -494,7 +531,7 @@ float rough_reflect_dist(v3 d, v3 next, const Surface& s, const Mat& m) {  // di
   IOR rel = rel_ior(raw, d, m);
   float cmp = fmx(dot(-d, macro), 1e-6f);
   float nmp = fmx(dot(next, macro), 1e-6f);
-  float rd = ggx_d(micro, macro, m.alpha) * ggx_g2(std::acos(cmp), std::acos(nmp), m.alpha) /
+  float rd = ggx_d(micro, macro, m.alpha) * ggx_g2(cmp, nmp, m.alpha) /
              (4.0f * cmp * nmp);
   return fresnel(d, micro, rel) * rd * np;
 }
@@ -505,7 +542,7 @@ inline float trans_ior(IOR i) { return 1.0f / rel_value(i); }
 float transmit_pdf(v3 d, v3 micro, v3 macro, v3 next, float tior, float alpha) {
   float cmp = fmx(dot(-d, macro), 1e-6f);
   float cmi = fmx(dot(-d, micro), 1e-6f);
-  float vnp = ggx_d(micro, macro, alpha) * ggx_g1(std::acos(cmp), alpha) * cmi / cmp;
+  float vnp = ggx_d(micro, macro, alpha) * ggx_g1(cmp, alpha) * cmi / cmp;
   float nmi = dot(next, micro);
   float den = nmi + cmi / tior;
   return vnp * cmi / fmx(den * den, 1e-6f);
@@ -548,7 +585,7 @@ float rough_trans_dist(v3 d, v3 next, const Surface& s, const Mat& m) {
   float nmi = dot(next, micro);
   float den = nmi + cmi / tior;
   float td = m.transmission * ggx_d(micro, macro, m.alpha) *
-             ggx_g2(std::acos(cmp), std::acos(nmp), m.alpha) * std::fabs(cmi * nmi) /
+             ggx_g2(cmp, nmp, m.alpha) * std::fabs(cmi * nmi) /
              fmx(cmp * nmp * den * den, 1e-6f);
   return (1.0f - fresnel(d, micro, rel)) * td * np;
 }
@@ -950,7 +987,7 @@ inline void spawn(const Surface& s, v3 next, v3& pos, v3& dir) {
 }
 
 // environment.wesl:7-25
-float env_radiance(const orc_scene& sc, v3 dir, float l) {
+float env_radiance(const orc_scene& sc, v3 dir, float l, uint32_t* texel) {
   v3 d = normalize(dir);
   float u = std::atan2(d.y, d.x) / (2.0f * PI) + 0.5f;
   float v = std::acos(clampf(d.z, -1.0f, 1.0f)) / PI;
@@ -959,6 +996,7 @@ float env_radiance(const orc_scene& sc, v3 dir, float l) {
   uint32_t x = std::min(sc.env_w - 1u, f2u(wu * static_cast<float>(sc.env_w)));
   uint32_t y = std::min(sc.env_h - 1u, f2u(cv * static_cast<float>(sc.env_h)));
   const float* t = &sc.env[(static_cast<size_t>(y) * sc.env_w + x) * 4];
+  *texel = y * sc.env_w + x;
   return rgb_to_radiance(mk(t[0], t[1], t[2]), l);
 }
 
@@ -979,7 +1017,12 @@ struct Tracer {
     Hit h = ray_hit(sc, pos, dir, cnt, cull, nullptr, nullptr);
     rays++;
     hash = hmix(hmix(hash, h.inst), h.tri);
-    if (!h.hit) return env_radiance(sc, dir, lambda);
+    if (!h.hit) {
+      uint32_t texel = 0;
+      float r = env_radiance(sc, dir, lambda, &texel);
+      hash = hmix(hash, 0x5E000000u ^ texel);  // the sky texel is a discrete decision too
+      return r;
+    }
     int trace_depth = std::max(1, std::min(P.D, 64));
     if (depth >= P.D || depth >= trace_depth - 1) return 0.0f;
     Ctx x = forward_pass(sc, dir, lambda, h.s, depth, P, rng);
@@ -1108,6 +1151,8 @@ void blas_build_tree(orc_scene& sc, uint32_t node_idx, uint32_t l, uint32_t r, s
 }  // namespace
 
 extern "C" {
+
+void orc_sincos(float x, float* s, float* c) { sincos_decl(x, *s, *c); }
 
 orc_scene* orc_scene_build(const cg_vertex* vertices, uint32_t vertex_count, const uint32_t* indices,
                            uint32_t index_count, const cg_material* materials, uint32_t material_count,

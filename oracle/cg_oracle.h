@@ -15,7 +15,10 @@
  *   - f32 -> u32 saturates and maps NaN / negatives to 0;
  *   - runtime array indices are clamped to the array (WebGPU robustness);
  *   - pow(x, 2) = x*x and pow(x, 4) = (x*x)*(x*x);
- *   - exp/cos/sin/tan/acos/atan2/pow(x, 1/2.4) are the C library's.
+ *   - tan(acos(c))^2 in the GGX masking terms = max(0, 1 - c^2) / c^2;
+ *   - sin/cos of the sampling angles: Cody-Waite reduction and Taylor
+ *     polynomials (orc_sincos; about 1 ulp, inside WGSL's 2^-11 bound);
+ *   - exp/acos/atan2/pow(x, 1/2.4) are the C library's.
  */
 #ifndef CG_ORACLE_H_
 #define CG_ORACLE_H_
@@ -55,7 +58,8 @@ void orc_trace_closest(const orc_scene* s, const cg_ray* rays, uint32_t n,
 /* The compute pass over pixel rows [y0, y1) (compute.wesl:32-62).  history is
  * the full W*H buffer, read when iter_count > 0 and written for the rows.
  * rgba8 (nullable, W*H*4), path_hash (nullable, W*H) receives a hash of the
- * pixel's sequence of hit ids and shadow outcomes, ray_count (nullable, W*H)
+ * pixel's sequence of hit ids, shadow outcomes and sky texels, ray_count
+ * (nullable, W*H)
  * its closest-hit queries.  Returns 0, or -1 on an unsupported scene. */
 int orc_render(const orc_scene* s, const cg_conf* conf, const cg_camera* cam,
                uint32_t iter_count, cg_history_pixel* history, uint8_t* rgba8,
@@ -63,6 +67,7 @@ int orc_render(const orc_scene* s, const cg_conf* conf, const cg_camera* cam,
                uint32_t y0, uint32_t y1, int threads, int cull);
 
 /* Unit-level pieces, for known-answer and closed-form checks. */
+void orc_sincos(float x, float* s, float* c);
 uint32_t orc_rng_step(uint32_t state);
 float orc_rand_float(uint32_t* state);
 float orc_erf(float x);
