@@ -93,6 +93,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe-copy-gib", type=float, default=2.0,
+                    help="size of the plain-copy HBM probe run before the frames (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark measures one GPU; --gpus must be 1 (the path is single-GPU: "
@@ -114,6 +116,25 @@ def main() -> int:
     ds = env.upload(sd)
     stream = env.stream
     ev = HipEvents()
+
+    probe = None
+    if args.probe_copy_gib > 0:
+        nbytes = int(args.probe_copy_gib * (1 << 30)) // 16 * 16
+        src, dst = env.alloc(nbytes), env.alloc(nbytes)
+        env.stream_copy(dst, src, nbytes)  # first touch
+        env.sync()
+        a, b = ev.create(), ev.create()
+        ev.record(a, stream)
+        reps = 5
+        for _ in range(reps):
+            env.stream_copy(dst, src, nbytes)
+        ev.record(b, stream)
+        env.sync()
+        cms = ev.elapsed_ms(a, b) / reps
+        probe = {"kernel": "k_stream_copy", "bytes_read": nbytes, "bytes_written": nbytes,
+                 "ms": round(cms, 3), "GB/s": round(2 * nbytes / (cms * 1e-3) / 1e9, 1)}
+        src.free()
+        dst.free()
 
     # Untimed counted frame: rays and node/triangle/instance fetches per frame.
     env.set_counting(True)
@@ -139,6 +160,15 @@ def main() -> int:
 
     alg_bytes = RAY_IO_BYTES * rays + PAIR_BYTES * st["nodes"] + TRI_BYTES * st["tris"] + INST_BYTES * st["insts"]
     achieved = alg_bytes / (ms * 1e-3) / 1e9
+
+    # HBM bytes per launch come from the committed rocprofv3 --pmc runs of this
+    # workload (FETCH_SIZE / WRITE_SIZE, scaled by a copy kernel of known size).
+    traffic, traffic_src = None, None
+    prof = os.path.join(ROOT, "profiles", "k_view_c1_1280x720_16spp.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f)["traffic_bytes"]
+        traffic_src = os.path.relpath(prof, ROOT)
 
     base = None
     if not args.no_cpu:
@@ -171,12 +201,15 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "frac_of_measured": None if not probe else round(achieved / probe["GB/s"], 4),
+            "traffic": None if traffic is None else round(traffic),
+            "traffic_source": traffic_src,
             "kernel": "k_view",
             "per_launch": {"pair_nodes": st["nodes"], "triangles": st["tris"], "instances": st["insts"],
                            "queries": rays, "algorithmic_bytes": alg_bytes},
         },
         "cpu_baseline": base,
+        "hbm_probe": probe,
         "latency_ms": None,
     }
     print(json.dumps(out))

@@ -258,6 +258,9 @@ class DeviceEnv:
         a = np.ascontiguousarray(hist, HISTORY)
         self._check(native.device().cg_write_history(self._h, a.ctypes.data))
 
+    def stream_copy(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int) -> None:
+        self._check(native.device().cg_stream_copy(self._h, dst.ptr, src.ptr, nbytes))
+
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
 

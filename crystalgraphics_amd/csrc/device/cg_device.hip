@@ -182,6 +182,13 @@ __global__ __launch_bounds__(kBlock) void k_view(ViewArgs A) {
   if (A.stats) flush_stats(A.stats, nrays, nshadow, st.nodes, st.tris, st.insts);
 }
 
+// Plain streaming copy: 16 B per lane per iteration, grid-stride.
+__global__ __launch_bounds__(kBlock) void k_stream_copy(float4* __restrict__ dst, const float4* __restrict__ src,
+                                                        size_t n) {
+  size_t stride = static_cast<size_t>(gridDim.x) * kBlock;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 template <typename T>
 int upload_vec(cg_env* env, DevBuf& b, const std::vector<T>& v, uint64_t& total) {
   total += v.size() * sizeof(T);
@@ -496,6 +503,18 @@ int cg_trace_occluded(cg_env* env, cg_scene* scene, const cg_ray* rays, const fl
   else
     hipLaunchKernelGGL((k_trace<64, true>), grid, dim3(kBlock), 0, env->stream, scene->view, rays, tmax, n, nullptr,
                        nullptr, occluded, st);
+  CG_HIP(env, hipGetLastError());
+  return CG_OK;
+}
+
+int cg_stream_copy(cg_env* env, void* dst, const void* src, size_t bytes) {
+  if (!env || !dst || !src) return fail(env, CG_ERR_INVALID, "cg_stream_copy: null argument");
+  if (bytes % 16 != 0) return fail(env, CG_ERR_INVALID, "cg_stream_copy: size must be a multiple of 16");
+  const size_t n = bytes / 16;
+  if (n == 0) return CG_OK;
+  const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 256u * 64u);
+  hipLaunchKernelGGL(k_stream_copy, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, env->stream,
+                     static_cast<float4*>(dst), static_cast<const float4*>(src), n);
   CG_HIP(env, hipGetLastError());
   return CG_OK;
 }

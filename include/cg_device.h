@@ -106,6 +106,12 @@ int cg_trace_closest(cg_env* env, cg_scene* scene, const cg_ray* rays, uint32_t 
 int cg_trace_occluded(cg_env* env, cg_scene* scene, const cg_ray* rays, const float* tmax, uint32_t n,
                       uint32_t* occluded);
 
+/* Measurement probe (no reference counterpart): copy `bytes` (a multiple of
+ * 16) from src to dst, both device buffers, with a plain coalesced 16-byte
+ * vector copy on the env's stream.  Its rate is the achievable HBM
+ * bandwidth, and its counters the scale for hardware byte counters. */
+int cg_stream_copy(cg_env* env, void* dst, const void* src, size_t bytes);
+
 /* HBM buffers for the batch entry points. */
 int cg_device_alloc(cg_env* env, size_t bytes, void** dptr);
 int cg_device_free(cg_env* env, void* dptr);
