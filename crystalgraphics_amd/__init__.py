@@ -215,6 +215,12 @@ class DeviceEnv:
         del keep
         return DeviceScene(self, h.value)
 
+    ACCEL_REFERENCE, ACCEL_REBUILT = 0, 1
+
+    def set_accel(self, mode: int) -> None:
+        """Acceleration layout for later uploads (0: the reference's trees, 1: rebuilt)."""
+        self._check(native.device().cg_env_set_accel(self._h, int(mode)))
+
     def set_counting(self, on: bool = True) -> None:
         self._check(native.device().cg_env_set_counting(self._h, int(on)))
 

@@ -204,8 +204,7 @@ CG_HD HitRec traverse(const DevScene& S, f3 wo, f3 wd, float tmax, uint32_t* stk
   uint32_t cur_inst = CG_MISS_ID;
   int sp = 0;
   for (;;) {
-    uint32_t kind = ref & CG_REF_KIND_MASK;
-    if (kind == 0u) {
+    if ((ref & (CG_REF_LEAF | CG_REF_INST)) == 0u) {
       const PairNode* n = S.nodes + ref;
       st.nodes++;
       float tl = box_enter(o, inv, n->lo_l, n->hi_l);
@@ -226,7 +225,7 @@ CG_HD HitRec traverse(const DevScene& S, f3 wo, f3 wd, float tmax, uint32_t* stk
         ref = n->ref_r;
         continue;
       }
-    } else if (kind == CG_REF_INST) {
+    } else if ((ref & CG_REF_LEAF) == 0u) {
       uint32_t ii = ref & CG_REF_INDEX_MASK;
       const InstRec* in = S.insts + ii;
       st.insts++;
@@ -241,8 +240,8 @@ CG_HD HitRec traverse(const DevScene& S, f3 wo, f3 wd, float tmax, uint32_t* stk
         continue;
       }
     } else {
-      uint32_t slot = ref & CG_REF_INDEX_MASK & ~CG_REF_LEAF2;
-      uint32_t cnt = (ref & CG_REF_LEAF2) ? 2u : 1u;
+      const uint32_t slot = ref & CG_REF_SLOT_MASK;
+      const uint32_t cnt = ((ref >> CG_REF_COUNT_SHIFT) & 3u) + 1u;
       for (uint32_t i = 0; i < cnt; ++i) {
         const TriRec* tr = S.tris + slot + i;
         st.tris++;
@@ -251,7 +250,7 @@ CG_HD HitRec traverse(const DevScene& S, f3 wo, f3 wd, float tmax, uint32_t* stk
           best.t = h.t;
           best.u = h.u;
           best.v = h.v;
-          best.inst = cur_inst;
+          best.inst = tr->inst != CG_MISS_ID ? tr->inst : cur_inst;
           best.slot = slot + i;
           if (kAnyHit) return best;
         }

@@ -48,6 +48,7 @@ struct cg_env {
   float wl_avg = 0.0f;
   bool counting = false;
   bool path_records = false;
+  int accel = CG_ACCEL_REBUILT;
   std::vector<cg_scene*> scenes;
 };
 
@@ -320,6 +321,13 @@ int cg_env_set_path_records(cg_env* env, int enable) {
   return CG_OK;
 }
 
+int cg_env_set_accel(cg_env* env, int mode) {
+  if (!env) return fail(nullptr, CG_ERR_INVALID, "null env");
+  if (mode != CG_ACCEL_REFERENCE && mode != CG_ACCEL_REBUILT) return fail(env, CG_ERR_INVALID, "unknown accel mode %d", mode);
+  env->accel = mode;
+  return CG_OK;
+}
+
 int cg_stats_read(cg_env* env, cg_stats* out) {
   if (!env || !out) return fail(env, CG_ERR_INVALID, "null argument");
   unsigned long long v[8];
@@ -344,7 +352,7 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
   *out = nullptr;
   Layout L;
   std::string err;
-  int rc = build_layout(sd, L, err);
+  int rc = build_layout(sd, L, err, env->accel);
   if (rc != CG_OK) return fail(env, rc, "%s", err.c_str());
   if (L.stack_depth > 64u)
     return fail(env, CG_ERR_UNSUPPORTED, "BVH needs a %u-entry traversal stack (max 64)", L.stack_depth);

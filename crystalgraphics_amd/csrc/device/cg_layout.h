@@ -5,7 +5,8 @@
 // multiple of 16 B, so one traversal step is a handful of dwordx4 loads from
 // one or two 128-byte lines:
 //   PairNode  64 B  both child boxes of one interior BVH node + child refs
-//   TriRec    48 B  v0, e1 = v1 - v0, e2 = v2 - v0, source triangle id
+//   TriRec    48 B  v0, e1 = v1 - v0, e2 = v2 - v0, source triangle id (and,
+//                   in the merged world BLAS, the triangle's instance)
 //   InstRec   80 B  full column-major inverse transform, BLAS root ref, material
 //   VAttr     32 B  normal + both uv sets (read once, for the final hit)
 //   MatRec    64 B  the wavelength-independent part of a decoded material
@@ -16,18 +17,20 @@
 #include <stdint.h>
 
 // Child / root references (32 bits):
-//   00 | index   interior pair node
-//   01 | index   TLAS leaf: instance index
-//   1c | slot    BLAS leaf: first triangle slot, c = (triangle count - 1)
-//   kRefNone     empty (BLAS with no triangles, or empty TLAS)
-//   kRefMarker   traversal-stack marker: return from a BLAS to the TLAS
-#define CG_REF_KIND_MASK 0xC0000000u
-#define CG_REF_INDEX_MASK 0x3FFFFFFFu
-#define CG_REF_INST 0x40000000u
+//   00 | index       interior pair node (30-bit index)
+//   01 | index       TLAS leaf: instance index (30-bit)
+//   1 cc | slot      BLAS leaf: cc + 1 triangles (1-4) from slot (29-bit)
+//   kRefNone         empty (BLAS with no triangles, or empty TLAS)
+//   kRefMarker       traversal-stack marker: return from a BLAS to the TLAS
 #define CG_REF_LEAF 0x80000000u
-#define CG_REF_LEAF2 0x40000000u
+#define CG_REF_INST 0x40000000u
+#define CG_REF_INDEX_MASK 0x3FFFFFFFu
+#define CG_REF_SLOT_MASK 0x1FFFFFFFu
+#define CG_REF_COUNT_SHIFT 29
 #define CG_REF_NONE 0x3FFFFFFFu
 #define CG_REF_MARKER 0x7FFFFFFFu
+#define CG_LEAF_MAX 4u
+#define CG_MAKE_LEAF(slot, count) (CG_REF_LEAF | (((count) - 1u) << CG_REF_COUNT_SHIFT) | (slot))
 
 struct __attribute__((aligned(16))) PairNode {
   float lo_l[3];   // left child box
@@ -44,7 +47,8 @@ struct __attribute__((aligned(16))) TriRec {
   float e1[3];
   float e2[3];
   uint32_t tri_id;
-  uint32_t _pad[2];
+  uint32_t inst;  // instance of a triangle merged into the world BLAS, else CG_MISS_ID
+  uint32_t _pad;
 };
 
 struct __attribute__((aligned(16))) InstRec {

@@ -142,7 +142,9 @@ struct Layout {
   uint32_t stack_depth = 0;
 };
 
-inline int build_layout(const cg_scene_data* sd, Layout& L, std::string& err) {
+// Wavelength-independent records shared by both acceleration layouts:
+// materials, vertex attributes, emitters, alias table; plus input checks.
+inline int build_common(const cg_scene_data* sd, Layout& L, std::string& err) {
   // Materials: textures are not implemented by the kernels.
   for (uint32_t i = 0; i < sd->material_count; ++i) {
     const cg_material& m = sd->materials[i];
@@ -168,7 +170,7 @@ inline int build_layout(const cg_scene_data* sd, Layout& L, std::string& err) {
   if (sd->alias_count != sd->emissive_count)
     return set_error(err, CG_ERR_INVALID, "alias table (%u) and emissive buffer (%u) differ in length", sd->alias_count,
                 sd->emissive_count);
-  if (sd->index_count >= (1u << 30) - 1u) return set_error(err, CG_ERR_INVALID, "too many triangles");
+  if (sd->index_count >= CG_REF_SLOT_MASK) return set_error(err, CG_ERR_INVALID, "too many triangles");
 
   // Vertices: attributes read for the final hit only.
   L.vattr.resize(sd->vertex_count);
@@ -182,24 +184,66 @@ inline int build_layout(const cg_scene_data* sd, Layout& L, std::string& err) {
     a.uv1[1] = v.uv1[1];
     a._pad = 0.0f;
   }
-  // Triangles in BVH order.
-  L.tris.resize(sd->index_count);
-  L.vidx.resize(sd->index_count);
   for (uint32_t s = 0; s < sd->index_count; ++s) {
     const cg_index& ix = sd->indices[s];
     if (ix.i0 >= sd->vertex_count || ix.i1 >= sd->vertex_count || ix.i2 >= sd->vertex_count)
       return set_error(err, CG_ERR_INVALID, "triangle slot %u indexes past the vertex buffer", s);
-    const float* p0 = sd->vertices[ix.i0].pos;
-    const float* p1 = sd->vertices[ix.i1].pos;
-    const float* p2 = sd->vertices[ix.i2].pos;
-    TriRec& t = L.tris[s];
+  }
+  // Emitters and alias table.
+  for (uint32_t i = 0; i < sd->emissive_count; ++i) {
+    const cg_emissive_prim& e = sd->emissive[i];
+    EmitRec r{};
     for (int k = 0; k < 3; ++k) {
-      t.v0[k] = p0[k];
-      t.e1[k] = p1[k] - p0[k];
-      t.e2[k] = p2[k] - p0[k];
+      r.v0[k] = e.v[0][k];
+      r.v1[k] = e.v[1][k];
+      r.v2[k] = e.v[2][k];
+      r.rgb[k] = e.emission[k];
     }
-    t.tri_id = sd->tri_ids ? sd->tri_ids[s] : s;
-    t._pad[0] = t._pad[1] = 0u;
+    f3 p0 = ld3(e.v[0]), p1 = ld3(e.v[1]), p2 = ld3(e.v[2]);
+    f3 ln = normalize(cross(p1 - p0, p2 - p0));
+    r.ln_x = ln.x;
+    r.ln_y = ln.y;
+    r.ln_z = ln.z;
+    r.area_pdf = fmaxf(e.weight / e.area, 1e-3f);
+    if (!std::isfinite(r.area_pdf) || !std::isfinite(ln.x))
+      return set_error(err, CG_ERR_INVALID, "emissive triangle %u is degenerate", i);
+    L.emit.push_back(r);
+  }
+  for (uint32_t i = 0; i < sd->alias_count; ++i) {
+    float2 a;
+    uint32_t al = sd->alias[i].alias;
+    std::memcpy(&a.x, &al, 4);
+    a.y = sd->alias[i].prob;
+    L.alias.push_back(a);
+  }
+  return CG_OK;
+}
+
+inline TriRec make_tri(const cg_scene_data* sd, uint32_t s, uint32_t inst) {
+  const cg_index& ix = sd->indices[s];
+  const float* p0 = sd->vertices[ix.i0].pos;
+  const float* p1 = sd->vertices[ix.i1].pos;
+  const float* p2 = sd->vertices[ix.i2].pos;
+  TriRec t;
+  for (int k = 0; k < 3; ++k) {
+    t.v0[k] = p0[k];
+    t.e1[k] = p1[k] - p0[k];
+    t.e2[k] = p2[k] - p0[k];
+  }
+  t.tri_id = sd->tri_ids ? sd->tri_ids[s] : s;
+  t.inst = inst;
+  t._pad = 0u;
+  return t;
+}
+
+// The reference's own trees (median split, leaves of 1-2 triangles), one pair
+// node per interior node: TLAS pairs first, BLAS pairs after.
+inline int build_reference_accel(const cg_scene_data* sd, Layout& L, std::string& err) {
+  L.tris.resize(sd->index_count);
+  L.vidx.resize(sd->index_count);
+  for (uint32_t s = 0; s < sd->index_count; ++s) {
+    L.tris[s] = make_tri(sd, s, CG_MISS_ID);
+    const cg_index& ix = sd->indices[s];
     L.vidx[s] = make_uint4(ix.i0, ix.i1, ix.i2, 0u);
   }
   // BLAS: one pair node per interior node (triangle_count == 0).
@@ -213,7 +257,7 @@ inline int build_layout(const cg_scene_data* sd, Layout& L, std::string& err) {
     if (n.triangle_count == 0u) continue;
     if (n.triangle_count > 2u || n.child + n.triangle_count > sd->index_count)
       return set_error(err, CG_ERR_INVALID, "BLAS leaf %u is malformed", i);
-    blas_ref[i] = CG_REF_LEAF | (n.triangle_count == 2u ? CG_REF_LEAF2 : 0u) | n.child;
+    blas_ref[i] = CG_MAKE_LEAF(n.child, n.triangle_count);
   }
   // TLAS pair nodes come first; BLAS pair nodes follow at offset `tp`.
   const uint32_t nt = sd->tlas_node_count;
@@ -297,36 +341,371 @@ inline int build_layout(const cg_scene_data* sd, Layout& L, std::string& err) {
     }
   }
   L.stack_depth = dt + dbl + 2u;
-  // Emitters and alias table.
-  for (uint32_t i = 0; i < sd->emissive_count; ++i) {
-    const cg_emissive_prim& e = sd->emissive[i];
-    EmitRec r{};
-    for (int k = 0; k < 3; ++k) {
-      r.v0[k] = e.v[0][k];
-      r.v1[k] = e.v[1][k];
-      r.v2[k] = e.v[2][k];
-      r.rgb[k] = e.emission[k];
-    }
-    f3 p0 = ld3(e.v[0]), p1 = ld3(e.v[1]), p2 = ld3(e.v[2]);
-    f3 ln = normalize(cross(p1 - p0, p2 - p0));
-    r.ln_x = ln.x;
-    r.ln_y = ln.y;
-    r.ln_z = ln.z;
-    r.area_pdf = fmaxf(e.weight / e.area, 1e-3f);
-    if (!std::isfinite(r.area_pdf) || !std::isfinite(ln.x))
-      return set_error(err, CG_ERR_INVALID, "emissive triangle %u is degenerate", i);
-    L.emit.push_back(r);
-  }
-  for (uint32_t i = 0; i < sd->alias_count; ++i) {
-    float2 a;
-    uint32_t al = sd->alias[i].alias;
-    std::memcpy(&a.x, &al, 4);
-    a.y = sd->alias[i].prob;
-    L.alias.push_back(a);
-  }
   return CG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Binned SAH build (16 bins per axis) over primitive boxes; leaves of at most
+// `max_leaf` primitives.  Nodes are stored parents first, left subtree before
+// right (depth-first), so a pair node's left child follows it in memory.
+struct SahTree {
+  struct Prim {
+    float lo[3], hi[3], c[3];
+  };
+  struct Node {
+    float lo[3], hi[3];
+    uint32_t left = 0, right = 0, first = 0, count = 0;  // leaf iff count > 0
+  };
+  std::vector<Node> nodes;
+  std::vector<uint32_t> order;  // primitive ids in leaf order
+  uint32_t depth = 0;
+
+  static double area(const float* lo, const float* hi) {
+    double dx = double(hi[0]) - lo[0], dy = double(hi[1]) - lo[1], dz = double(hi[2]) - lo[2];
+    if (dx < 0 || dy < 0 || dz < 0) return 0.0;
+    return 2.0 * (dx * dy + dy * dz + dz * dx);
+  }
+
+  void build(const std::vector<Prim>& prims, uint32_t max_leaf) {
+    const uint32_t n = static_cast<uint32_t>(prims.size());
+    order.resize(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    nodes.clear();
+    nodes.reserve(n ? 2 * n : 1);
+    if (n == 0) return;
+    struct Task {
+      uint32_t node, first, count, depth;
+    };
+    nodes.emplace_back();
+    std::vector<Task> stack{{0u, 0u, n, 0u}};
+    constexpr int kBins = 16;
+    while (!stack.empty()) {
+      Task t = stack.back();
+      stack.pop_back();
+      depth = std::max(depth, t.depth);
+      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (uint32_t i = t.first; i < t.first + t.count; ++i) {
+        const Prim& p = prims[order[i]];
+        for (int k = 0; k < 3; ++k) {
+          lo[k] = std::min(lo[k], p.lo[k]);
+          hi[k] = std::max(hi[k], p.hi[k]);
+          clo[k] = std::min(clo[k], p.c[k]);
+          chi[k] = std::max(chi[k], p.c[k]);
+        }
+      }
+      Node& nd = nodes[t.node];
+      for (int k = 0; k < 3; ++k) {
+        nd.lo[k] = lo[k];
+        nd.hi[k] = hi[k];
+      }
+      auto make_leaf = [&]() {
+        nodes[t.node].first = t.first;
+        nodes[t.node].count = t.count;
+      };
+      if (t.count <= 1) {
+        make_leaf();
+        continue;
+      }
+      // best split over three axes x 15 planes
+      double best_cost = INFINITY;
+      int best_axis = -1, best_plane = 0;
+      for (int axis = 0; axis < 3; ++axis) {
+        const float ext = chi[axis] - clo[axis];
+        if (!(ext > 0.0f)) continue;
+        const double scale = kBins / double(ext);
+        uint32_t cnt[kBins] = {};
+        float blo[kBins][3], bhi[kBins][3];
+        for (int b = 0; b < kBins; ++b)
+          for (int k = 0; k < 3; ++k) {
+            blo[b][k] = INFINITY;
+            bhi[b][k] = -INFINITY;
+          }
+        for (uint32_t i = t.first; i < t.first + t.count; ++i) {
+          const Prim& p = prims[order[i]];
+          int b = std::min(kBins - 1, static_cast<int>((p.c[axis] - clo[axis]) * scale));
+          cnt[b]++;
+          for (int k = 0; k < 3; ++k) {
+            blo[b][k] = std::min(blo[b][k], p.lo[k]);
+            bhi[b][k] = std::max(bhi[b][k], p.hi[k]);
+          }
+        }
+        double right_area[kBins];
+        uint32_t right_cnt[kBins];
+        float rlo[3] = {INFINITY, INFINITY, INFINITY}, rhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        uint32_t rc = 0;
+        for (int b = kBins - 1; b > 0; --b) {
+          rc += cnt[b];
+          for (int k = 0; k < 3; ++k) {
+            rlo[k] = std::min(rlo[k], blo[b][k]);
+            rhi[k] = std::max(rhi[k], bhi[b][k]);
+          }
+          right_area[b] = area(rlo, rhi);
+          right_cnt[b] = rc;
+        }
+        float llo[3] = {INFINITY, INFINITY, INFINITY}, lhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        uint32_t lc = 0;
+        for (int b = 0; b < kBins - 1; ++b) {
+          lc += cnt[b];
+          for (int k = 0; k < 3; ++k) {
+            llo[k] = std::min(llo[k], blo[b][k]);
+            lhi[k] = std::max(lhi[k], bhi[b][k]);
+          }
+          if (lc == 0 || right_cnt[b + 1] == 0) continue;
+          double c = area(llo, lhi) * lc + right_area[b + 1] * right_cnt[b + 1];
+          if (c < best_cost) {
+            best_cost = c;
+            best_axis = axis;
+            best_plane = b;
+          }
+        }
+      }
+      const double parent_area = std::max(area(lo, hi), 1e-30);
+      const double split_cost = 1.0 + best_cost / parent_area;  // traversal 1, intersection 1 per primitive
+      if (t.count <= max_leaf && (best_axis < 0 || double(t.count) <= split_cost)) {
+        make_leaf();
+        continue;
+      }
+      uint32_t mid;
+      if (best_axis < 0) {  // all centroids equal: split by count
+        mid = t.first + t.count / 2;
+      } else {
+        const double scale = kBins / double(chi[best_axis] - clo[best_axis]);
+        const float c0 = clo[best_axis];
+        auto it = std::partition(order.begin() + t.first, order.begin() + t.first + t.count, [&](uint32_t id) {
+          int b = std::min(kBins - 1, static_cast<int>((prims[id].c[best_axis] - c0) * scale));
+          return b <= best_plane;
+        });
+        mid = static_cast<uint32_t>(it - order.begin());
+        if (mid == t.first || mid == t.first + t.count) mid = t.first + t.count / 2;
+      }
+      const uint32_t left = static_cast<uint32_t>(nodes.size());
+      nodes.emplace_back();
+      nodes.emplace_back();
+      nodes[t.node].left = left;
+      nodes[t.node].right = left + 1;
+      // right pushed first so the left subtree is processed (and numbered) first
+      stack.push_back({left + 1, mid, t.first + t.count - mid, t.depth + 1});
+      stack.push_back({left, t.first, mid - t.first, t.depth + 1});
+    }
+  }
+};
+
+// Pair nodes for a SAH tree: one per interior node, appended to `out`.
+// Leaves become BLAS leaf refs (slot_base + first) or, for a TLAS
+// (`inst_ids` given), instance refs.  Returns the root ref.
+inline uint32_t emit_pairs(const SahTree& t, std::vector<PairNode>& out, uint32_t slot_base,
+                           const std::vector<uint32_t>* inst_ids, float pad) {
+  if (t.nodes.empty()) return CG_REF_NONE;
+  std::vector<uint32_t> pair_of(t.nodes.size(), CG_REF_NONE);
+  const uint32_t base = static_cast<uint32_t>(out.size());
+  uint32_t next = base;
+  for (size_t i = 0; i < t.nodes.size(); ++i)
+    if (t.nodes[i].count == 0) pair_of[i] = next++;
+  out.resize(next);
+  auto ref = [&](uint32_t i) -> uint32_t {
+    const SahTree::Node& n = t.nodes[i];
+    if (n.count == 0) return pair_of[i];
+    if (inst_ids) return CG_REF_INST | (*inst_ids)[t.order[n.first]];
+    return CG_MAKE_LEAF(slot_base + n.first, n.count);
+  };
+  for (size_t i = 0; i < t.nodes.size(); ++i) {
+    const SahTree::Node& n = t.nodes[i];
+    if (n.count != 0) continue;
+    PairNode& p = out[pair_of[i]];
+    const SahTree::Node& l = t.nodes[n.left];
+    const SahTree::Node& r = t.nodes[n.right];
+    for (int k = 0; k < 3; ++k) {
+      p.lo_l[k] = l.lo[k] - pad;
+      p.hi_l[k] = l.hi[k] + pad;
+      p.lo_r[k] = r.lo[k] - pad;
+      p.hi_r[k] = r.hi[k] + pad;
+    }
+    p.ref_l = ref(n.left);
+    p.ref_r = ref(n.right);
+    p._pad[0] = p._pad[1] = 0u;
+  }
+  return ref(0);
+}
+
+// Rebuilt acceleration structure (same closest hits, ties aside):
+//  * every distinct BLAS rebuilt with binned SAH, leaves of up to 4 triangles;
+//  * instances whose inverse transform is the identity and whose BLAS holds at
+//    most 16 triangles (e.g. one-triangle emitters) merged into one world-space
+//    BLAS whose triangles carry their instance index -- their local rays equal
+//    the world rays, so the same triangle tests run on the same rays;
+//  * a SAH TLAS over the remaining instances and the merged BLAS.
+inline int build_sah_accel(const cg_scene_data* sd, Layout& L, std::string& err) {
+  const uint32_t nb = sd->blas_node_count, ni = sd->instance_count, nt = sd->tlas_node_count;
+  // World boxes of the instances: the reference TLAS leaves.
+  std::vector<int> have_box(ni, 0);
+  std::vector<SahTree::Prim> inst_box(ni);
+  for (uint32_t i = 0; i < nt; ++i) {
+    const cg_tlas_node& n = sd->tlas_nodes[i];
+    if (n.child != 0u) {
+      if (n.child + 1u >= nt) return set_error(err, CG_ERR_INVALID, "TLAS node %u has children past the end", i);
+      continue;
+    }
+    if (n.instance_idx >= ni)
+      return set_error(err, CG_ERR_INVALID, "TLAS leaf %u names instance %u of %u", i, n.instance_idx, ni);
+    SahTree::Prim& p = inst_box[n.instance_idx];
+    for (int k = 0; k < 3; ++k) {
+      p.lo[k] = n.lb[k];
+      p.hi[k] = n.ub[k];
+      p.c[k] = 0.5f * (n.lb[k] + n.ub[k]);
+    }
+    have_box[n.instance_idx] = 1;
+  }
+  // Triangle slots of each BLAS root (walk the reference tree).
+  auto slots_of = [&](uint32_t root, std::vector<uint32_t>& out) -> bool {
+    std::vector<uint32_t> st{root};
+    while (!st.empty()) {
+      uint32_t i = st.back();
+      st.pop_back();
+      if (i >= nb) return false;
+      const cg_blas_node& n = sd->blas_nodes[i];
+      if (n.triangle_count == 0u) {
+        if (n.child + 1u >= nb) return false;
+        st.push_back(n.child + 1u);
+        st.push_back(n.child);
+      } else {
+        if (n.triangle_count > 2u || n.child + n.triangle_count > sd->index_count) return false;
+        for (uint32_t k = 0; k < n.triangle_count; ++k) out.push_back(n.child + k);
+      }
+    }
+    return true;
+  };
+  auto is_identity = [](const float* m) {
+    for (int c = 0; c < 4; ++c)
+      for (int r = 0; r < 4; ++r)
+        if (m[4 * c + r] != (c == r ? 1.0f : 0.0f)) return false;
+    return true;
+  };
+  auto tri_prim = [&](uint32_t s) {
+    const cg_index& ix = sd->indices[s];
+    const float* p[3] = {sd->vertices[ix.i0].pos, sd->vertices[ix.i1].pos, sd->vertices[ix.i2].pos};
+    SahTree::Prim q;
+    for (int k = 0; k < 3; ++k) {
+      q.lo[k] = std::min(std::min(p[0][k], p[1][k]), p[2][k]);
+      q.hi[k] = std::max(std::max(p[0][k], p[1][k]), p[2][k]);
+      q.c[k] = 0.5f * (q.lo[k] + q.hi[k]);
+    }
+    return q;
+  };
+  // Box padding: a small multiple of the coordinates' magnitude keeps every
+  // box test conservative against the rounding of the triangle test.
+  auto pad_of = [](const std::vector<SahTree::Prim>& prims) {
+    float m = 1.0f;
+    for (const auto& p : prims)
+      for (int k = 0; k < 3; ++k) m = std::max(m, std::max(std::fabs(p.lo[k]), std::fabs(p.hi[k])));
+    return 4e-5f * m;
+  };
+
+  L.insts.resize(ni);
+  std::vector<uint32_t> merged, kept;
+  std::vector<std::vector<uint32_t>> root_slots;
+  std::vector<int> root_index(nb, -1);
+  for (uint32_t i = 0; i < ni; ++i) {
+    const cg_instance& in = sd->instances[i];
+    InstRec& r = L.insts[i];
+    std::memcpy(r.inv, in.inv_trans, sizeof(r.inv));
+    r.material = in.material_idx;
+    r._pad[0] = r._pad[1] = 0u;
+    r.root = CG_REF_NONE;
+    if (in.blas_root_idx >= nb) return set_error(err, CG_ERR_INVALID, "instance %u names BLAS root %u", i, in.blas_root_idx);
+    if (!have_box[i]) continue;  // not reachable from the TLAS
+    if (root_index[in.blas_root_idx] < 0) {
+      root_index[in.blas_root_idx] = static_cast<int>(root_slots.size());
+      root_slots.emplace_back();
+      if (!slots_of(in.blas_root_idx, root_slots.back()))
+        return set_error(err, CG_ERR_INVALID, "BLAS rooted at %u is malformed", in.blas_root_idx);
+    }
+    const auto& sl = root_slots[root_index[in.blas_root_idx]];
+    if (is_identity(in.inv_trans) && sl.size() <= 16u)
+      merged.push_back(i);
+    else
+      kept.push_back(i);
+  }
+  uint32_t max_blas_depth = 0;
+  // Distinct BLAS of kept instances.
+  std::vector<uint32_t> built_root(root_slots.size(), CG_REF_NONE);
+  for (uint32_t i : kept) {
+    const int ri = root_index[sd->instances[i].blas_root_idx];
+    if (built_root[ri] == CG_REF_NONE && !root_slots[ri].empty()) {
+      const auto& sl = root_slots[ri];
+      std::vector<SahTree::Prim> prims(sl.size());
+      for (size_t k = 0; k < sl.size(); ++k) prims[k] = tri_prim(sl[k]);
+      SahTree t;
+      t.build(prims, CG_LEAF_MAX);
+      const uint32_t slot_base = static_cast<uint32_t>(L.tris.size());
+      for (uint32_t id : t.order) {
+        L.tris.push_back(make_tri(sd, sl[id], CG_MISS_ID));
+        const cg_index& ix = sd->indices[sl[id]];
+        L.vidx.push_back(make_uint4(ix.i0, ix.i1, ix.i2, 0u));
+      }
+      built_root[ri] = 0;  // placeholder until pair nodes are placed
+      const uint32_t r = emit_pairs(t, L.nodes, slot_base, nullptr, pad_of(prims));
+      built_root[ri] = r;
+      max_blas_depth = std::max(max_blas_depth, t.depth);
+    }
+    L.insts[i].root = built_root[ri];
+  }
+  // Merged world BLAS.
+  std::vector<SahTree::Prim> tlas_prims;
+  std::vector<uint32_t> tlas_ids;
+  for (uint32_t i : kept) {
+    if (L.insts[i].root == CG_REF_NONE) continue;
+    tlas_prims.push_back(inst_box[i]);
+    tlas_ids.push_back(i);
+  }
+  if (!merged.empty()) {
+    std::vector<SahTree::Prim> prims;
+    std::vector<std::pair<uint32_t, uint32_t>> tri_inst;  // (slot, instance)
+    for (uint32_t i : merged)
+      for (uint32_t s : root_slots[root_index[sd->instances[i].blas_root_idx]]) {
+        prims.push_back(tri_prim(s));
+        tri_inst.emplace_back(s, i);
+      }
+    SahTree t;
+    t.build(prims, CG_LEAF_MAX);
+    const uint32_t slot_base = static_cast<uint32_t>(L.tris.size());
+    for (uint32_t id : t.order) {
+      L.tris.push_back(make_tri(sd, tri_inst[id].first, tri_inst[id].second));
+      const cg_index& ix = sd->indices[tri_inst[id].first];
+      L.vidx.push_back(make_uint4(ix.i0, ix.i1, ix.i2, 0u));
+    }
+    const float pad = pad_of(prims);
+    InstRec w{};
+    for (int k = 0; k < 16; ++k) w.inv[k] = (k % 5 == 0) ? 1.0f : 0.0f;
+    w.root = emit_pairs(t, L.nodes, slot_base, nullptr, pad);
+    w.material = 0u;
+    const uint32_t wi = static_cast<uint32_t>(L.insts.size());
+    L.insts.push_back(w);
+    SahTree::Prim box;
+    for (int k = 0; k < 3; ++k) {
+      box.lo[k] = t.nodes[0].lo[k] - pad;
+      box.hi[k] = t.nodes[0].hi[k] + pad;
+      box.c[k] = 0.5f * (box.lo[k] + box.hi[k]);
+    }
+    tlas_prims.push_back(box);
+    tlas_ids.push_back(wi);
+    max_blas_depth = std::max(max_blas_depth, t.depth);
+  }
+  if (L.tris.size() >= CG_REF_SLOT_MASK) return set_error(err, CG_ERR_INVALID, "too many triangles");
+  // TLAS over instance boxes (already padded), one instance per leaf.
+  SahTree tl;
+  tl.build(tlas_prims, 1u);
+  L.root = emit_pairs(tl, L.nodes, 0u, &tlas_ids, 0.0f);
+  L.stack_depth = tl.depth + max_blas_depth + 2u;
+  return CG_OK;
+}
+
+// accel 0: the reference's trees; accel 1: rebuilt (SAH, merged world BLAS).
+inline int build_layout(const cg_scene_data* sd, Layout& L, std::string& err, int accel = 1) {
+  int rc = build_common(sd, L, err);
+  if (rc != CG_OK) return rc;
+  return accel == 0 ? build_reference_accel(sd, L, err) : build_sah_accel(sd, L, err);
+}
 
 }  // namespace cg
 

@@ -180,6 +180,7 @@ def device() -> C.CDLL:
         _sig(d, "cg_env_set_counting", I32, P, I32)
         _sig(d, "cg_env_set_path_records", I32, P, I32)
         _sig(d, "cg_stats_read", I32, P, C.POINTER(Stats))
+        _sig(d, "cg_env_set_accel", I32, P, I32)
         _sig(d, "cg_stats_reset", I32, P)
         _sig(d, "cg_scene_upload", I32, P, C.POINTER(SceneDataC), C.POINTER(P))
         _sig(d, "cg_scene_release", I32, P, P)

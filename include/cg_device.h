@@ -71,6 +71,14 @@ void* cg_env_stream(cg_env* env);
 int cg_env_set_counting(cg_env* env, int enable);
 int cg_env_set_path_records(cg_env* env, int enable);
 int cg_stats_read(cg_env* env, cg_stats* out);  /* synchronises */
+/* Acceleration structure built by later cg_scene_upload calls:
+ *   CG_ACCEL_REFERENCE: the SceneData's own BLAS/TLAS trees, node for node;
+ *   CG_ACCEL_REBUILT (default): every distinct BLAS rebuilt with binned SAH
+ *     (leaves of up to 4 triangles), identity-transform instances of at most 16
+ *     triangles merged into one world-space BLAS, and a SAH TLAS.
+ * Both give the same closest hits (up to triangles tied at the same t). */
+enum { CG_ACCEL_REFERENCE = 0, CG_ACCEL_REBUILT = 1 };
+int cg_env_set_accel(cg_env* env, int mode);
 int cg_stats_reset(cg_env* env);
 
 /* Upload a flattened scene and build the device layout (pair nodes, triangle

@@ -22,11 +22,16 @@ def env(cg):
     e.close()
 
 
+@pytest.mark.parametrize("accel", [0, 1])
 @pytest.mark.parametrize("name,params", SCENES)
-def test_closest_hit_ids(cg, oracle, env, name, params):
+def test_closest_hit_ids(cg, oracle, env, name, params, accel):
+    """Both device layouts (the reference's trees, and the rebuilt SAH trees
+    with merged world-space instances) give the oracle's closest hits."""
     s = cg.HostScene.generate(name, seed=11, params=params)
     sd = s.scene_data()
+    env.set_accel(accel)
     ds = env.upload(sd)
+    env.set_accel(1)
     rays = s.rays(1 << 16, seed=2)
     hits, uv = env.trace_closest(ds, rays, with_uv=True)
     ref = oracle.OracleScene.wrap(sd.c).trace_closest(rays)
@@ -55,12 +60,13 @@ def test_occlusion_matches_closest(cg, oracle, env, name, params):
     ds.release()
 
 
-def _render_parity(cg, oracle, name, params, conf, frames=1, seed=13):
+def _render_parity(cg, oracle, name, params, conf, frames=1, seed=13, accel=1):
     s = cg.HostScene.generate(name, seed=seed, params=params)
     sd = s.scene_data()
     cam = s.describe()["camera"]
     orc = oracle.OracleScene.wrap(sd.c)
     with cg.DeviceEnv(conf) as env:
+        env.set_accel(accel)
         ds = env.upload(sd)
         env.set_path_records(True)
         hist_ref = None
@@ -94,6 +100,12 @@ def test_render_per_pixel(cg, oracle, name, params, w, h, spp, depth):
     assert np.array_equal(hist["rand_seed"][same], ref["history"]["rand_seed"][same])
     d = np.abs(rgba.astype(int) - ref["rgba"].astype(int))[same]
     assert d.max() <= 1
+
+
+def test_render_reference_layout(cg, oracle):
+    conf = cg.Conf.make(128, 72, 2, 8)
+    same, err, *_ = _render_parity(cg, oracle, "c1", [0.25], conf, accel=0)
+    assert same.mean() >= 0.99 and err[same].max() <= 1e-4
 
 
 def test_progressive_accumulation(cg, oracle):
