@@ -173,6 +173,8 @@ struct DevScene {
   uint32_t emit_count;
   const float2* alias;     // {bitcast alias index, prob}
   uint32_t alias_count;
+  const uint32_t* emit_order;  // emitter BVH leaf slot -> emitter index
+  uint32_t emit_root;          // emitter BVH root ref (CG_REF_NONE: none)
   const float4* env;
   uint32_t env_w, env_h;
   uint32_t root;           // TLAS root ref
@@ -674,10 +676,8 @@ CG_HD float emit_hit_dist(f3 pos, f3 dir, const EmitRec& e) {
 }
 // Solid-angle pdf of the emitters the ray from `pos` along `next` passes
 // through, summed in emitter order (emission.wesl:120-149).
-CG_HD float light_pdf(const DevScene& S, f3 next, f3 pos) {
-  if (S.alias_count == 0u) return 0.0f;
+CG_HD float light_pdf_linear(const DevScene& S, f3 rd, f3 pos) {
   float pdf = 0.0f;
-  f3 rd = normalize(next);
   for (uint32_t i = 0; i < S.emit_count; ++i) {
     const EmitRec& e = S.emit[i];
     float hd = emit_hit_dist(pos, rd, e);
@@ -685,6 +685,67 @@ CG_HD float light_pdf(const DevScene& S, f3 next, f3 pos) {
     f3 ln = mk3(e.ln_x, e.ln_y, e.ln_z);
     float lp = CG_FABS(dot(ln, -rd));
     pdf += e.area_pdf * hd * hd / fmx(lp, 1e-3f);
+  }
+  return pdf;
+}
+// The same sum, with the emitters the ray passes through found by a BVH over
+// the emitters (padded boxes, no distance cut-off) and added in ascending
+// emitter index, so the result equals the linear loop's bit for bit.  More
+// than kCap hits fall back to the linear loop.
+CG_HD float light_pdf(const DevScene& S, f3 next, f3 pos, uint32_t* stk, int stride) {
+  if (S.alias_count == 0u) return 0.0f;
+  const f3 rd = normalize(next);
+  if (S.emit_root == CG_REF_NONE) return light_pdf_linear(S, rd, pos);
+  constexpr int kCap = 8;
+  uint32_t idx[kCap];
+  float hds[kCap];
+  int n = 0;
+  const f3 inv = dir_inv(rd);
+  uint32_t ref = S.emit_root;
+  int sp = 0;
+  for (;;) {
+    if ((ref & CG_REF_LEAF) == 0u) {
+      const PairNode* nd = S.nodes + ref;
+      const bool hl = box_enter(pos, inv, nd->lo_l, nd->hi_l) < __builtin_inff();
+      const bool hr = box_enter(pos, inv, nd->lo_r, nd->hi_r) < __builtin_inff();
+      if (hl && hr) {
+        stk[sp * stride] = nd->ref_r;
+        ++sp;
+        ref = nd->ref_l;
+        continue;
+      }
+      if (hl || hr) {
+        ref = hl ? nd->ref_l : nd->ref_r;
+        continue;
+      }
+    } else {
+      const uint32_t slot = ref & CG_REF_SLOT_MASK;
+      const uint32_t cnt = ((ref >> CG_REF_COUNT_SHIFT) & 3u) + 1u;
+      for (uint32_t i = 0; i < cnt; ++i) {
+        const uint32_t e = S.emit_order[slot + i];
+        const float hd = emit_hit_dist(pos, rd, S.emit[e]);
+        if (hd <= 0.0f) continue;
+        if (n == kCap) return light_pdf_linear(S, rd, pos);
+        int k = n++;
+        while (k > 0 && idx[k - 1] > e) {
+          idx[k] = idx[k - 1];
+          hds[k] = hds[k - 1];
+          --k;
+        }
+        idx[k] = e;
+        hds[k] = hd;
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    ref = stk[sp * stride];
+  }
+  float pdf = 0.0f;
+  for (int k = 0; k < n; ++k) {
+    const EmitRec& e = S.emit[idx[k]];
+    f3 ln = mk3(e.ln_x, e.ln_y, e.ln_z);
+    float lp = CG_FABS(dot(ln, -rd));
+    pdf += e.area_pdf * hds[k] * hds[k] / fmx(lp, 1e-3f);
   }
   return pdf;
 }
@@ -761,13 +822,13 @@ enum { kNone = 0, kDiffuse, kSpecR, kRoughR, kSpecT, kRoughT };
 // The light pdf multiplies a zero count everywhere below the first bounce;
 // it is finite for finite emitters, so it is only evaluated when counted.
 CG_HD float balance_den(const DevScene& S, f3 d, f3 next, const Surface& s, const Mat& m, const Counts6& c,
-                        int model, float model_pdf, float light) {
+                        int model, float model_pdf, float light, uint32_t* stk, int stride) {
   float pd = model == kDiffuse ? model_pdf : diffuse_pdf(d, next, s);
   float psr = model == kSpecR ? model_pdf : spec_reflect_pdf(d, next, s);
   float prr = model == kRoughR ? model_pdf : rough_reflect_pdf(d, next, s, m);
   float pst = model == kSpecT ? model_pdf : spec_trans_pdf(d, next, s, m);
   float prt = model == kRoughT ? model_pdf : rough_trans_pdf(d, next, s, m);
-  float pl = light >= 0.0f ? light : (c.l != 0u ? light_pdf(S, next, s.pos) : 0.0f);
+  float pl = light >= 0.0f ? light : (c.l != 0u ? light_pdf(S, next, s.pos, stk, stride) : 0.0f);
   return fmx(static_cast<float>(c.d) * pd + static_cast<float>(c.sr) * psr + static_cast<float>(c.rr) * prr +
                  static_cast<float>(c.st) * pst + static_cast<float>(c.rt) * prt + static_cast<float>(c.l) * pl,
              1e-6f);
@@ -783,7 +844,7 @@ struct Frame5 {
   uint32_t tcount, ecount, iter;
 };
 CG_HD void forward_pass(const DevScene& S, f3 d, float l, const SmitsBin& b, const Surface& s, int depth,
-                        const TraceParams& P, uint32_t& rng, Frame5& x) {
+                        const TraceParams& P, uint32_t& rng, Frame5& x, uint32_t* stk, int stride) {
   const MatRec& pm = S.mats[umin(s.material, S.mat_count - 1u)];
   Mat m = decode(pm, l, b);
   Counts6 c = lobe_counts(m, S.alias_count != 0u, depth, P);
@@ -795,33 +856,33 @@ CG_HD void forward_pass(const DevScene& S, f3 d, float l, const SmitsBin& b, con
   for (uint32_t i = 0; i < c.d; ++i) {
     TSample t = diffuse_sample(d, s, rng);
     x.tdir[x.tcount] = t.dir;
-    x.tcoeff[x.tcount] = diffuse_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kDiffuse, t.prob, -1.0f);
+    x.tcoeff[x.tcount] = diffuse_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kDiffuse, t.prob, -1.0f, stk, stride);
     x.tcount++;
   }
   for (uint32_t i = 0; i < c.sr; ++i) {
     f3 nd = specular_dir(d, s);
     float prob = spec_reflect_pdf(d, nd, s);
     x.tdir[x.tcount] = nd;
-    x.tcoeff[x.tcount] = spec_reflect_dist(d, nd, s, m) / balance_den(S, d, nd, s, m, c, kSpecR, prob, -1.0f);
+    x.tcoeff[x.tcount] = spec_reflect_dist(d, nd, s, m) / balance_den(S, d, nd, s, m, c, kSpecR, prob, -1.0f, stk, stride);
     x.tcount++;
   }
   for (uint32_t i = 0; i < c.rr; ++i) {
     TSample t = rough_reflect_sample(d, s, m, rng);
     x.tdir[x.tcount] = t.dir;
-    x.tcoeff[x.tcount] = rough_reflect_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kRoughR, t.prob, -1.0f);
+    x.tcoeff[x.tcount] = rough_reflect_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kRoughR, t.prob, -1.0f, stk, stride);
     x.tcount++;
   }
   for (uint32_t i = 0; i < c.st; ++i) {
     f3 nd = spec_trans_dir(d, s, m);
     float prob = spec_trans_pdf(d, nd, s, m);
     x.tdir[x.tcount] = nd;
-    x.tcoeff[x.tcount] = spec_trans_dist(d, nd, s, m) / balance_den(S, d, nd, s, m, c, kSpecT, prob, -1.0f);
+    x.tcoeff[x.tcount] = spec_trans_dist(d, nd, s, m) / balance_den(S, d, nd, s, m, c, kSpecT, prob, -1.0f, stk, stride);
     x.tcount++;
   }
   for (uint32_t i = 0; i < c.rt; ++i) {
     TSample t = rough_trans_sample(d, s, m, rng);
     x.tdir[x.tcount] = t.dir;
-    x.tcoeff[x.tcount] = rough_trans_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kRoughT, t.prob, -1.0f);
+    x.tcoeff[x.tcount] = rough_trans_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kRoughT, t.prob, -1.0f, stk, stride);
     x.tcount++;
   }
   for (uint32_t i = 0; i < c.l; ++i) {
@@ -830,7 +891,7 @@ CG_HD void forward_pass(const DevScene& S, f3 d, float l, const SmitsBin& b, con
     float proj = CG_FABS(dot(s.sn, e.dir));
     float coeff = (diffuse_dist(d, e.dir, s, m) + rough_reflect_dist(d, e.dir, s, m)) + rough_trans_dist(d, e.dir, s, m);
     x.edir = e.dir;
-    x.erad = e.radiance * coeff * proj / balance_den(S, d, e.dir, s, m, c, kNone, -1.0f, e.prob);
+    x.erad = e.radiance * coeff * proj / balance_den(S, d, e.dir, s, m, c, kNone, -1.0f, e.prob, stk, stride);
     x.edist = e.dist;
     x.ecount++;
   }

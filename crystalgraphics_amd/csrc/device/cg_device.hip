@@ -32,7 +32,7 @@ struct DevBuf {
 }  // namespace
 
 struct cg_scene {
-  DevBuf nodes, tris, insts, vidx, vattr, mats, emit, alias, env;
+  DevBuf nodes, tris, insts, vidx, vattr, mats, emit, alias, emit_order, env;
   DevScene view{};
   uint32_t stack_depth = 0;
   uint64_t bytes = 0;
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const cg_ray* __re
 // the reference's depth-first order.  The transport tree's open frames live
 // in a private array (MF frames = trace_depth - 1); the BVH stack in LDS.
 template <int SS, int MF>
-__global__ __launch_bounds__(kBlock) void k_view(ViewArgs A) {
+__global__ __launch_bounds__(kBlock, 4) void k_view(ViewArgs A) {
   __shared__ uint32_t stk_lds[SS * kBlock];
   const uint32_t tiles_x = (A.W + 15u) / 16u;
   uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -205,6 +205,7 @@ void release_scene(cg_scene* s) {
   dev_free(s->mats);
   dev_free(s->emit);
   dev_free(s->alias);
+  dev_free(s->emit_order);
   dev_free(s->env);
 }
 
@@ -369,6 +370,7 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
   up(upload_vec(env, s->mats, L.mats, total));
   up(upload_vec(env, s->emit, L.emit, total));
   up(upload_vec(env, s->alias, L.alias, total));
+  up(upload_vec(env, s->emit_order, L.emit_order, total));
   size_t env_bytes = static_cast<size_t>(sd->env_width) * sd->env_height * 4 * sizeof(float);
   total += env_bytes;
   up(dev_upload(env, s->env, sd->env_rgba32f, env_bytes));
@@ -391,6 +393,8 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
   v.emit_count = static_cast<uint32_t>(L.emit.size());
   v.alias = static_cast<const float2*>(s->alias.p);
   v.alias_count = static_cast<uint32_t>(L.alias.size());
+  v.emit_order = static_cast<const uint32_t*>(s->emit_order.p);
+  v.emit_root = L.emit_root;
   v.env = static_cast<const float4*>(s->env.p);
   v.env_w = sd->env_width;
   v.env_h = sd->env_height;

@@ -74,7 +74,7 @@ CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, uint32_t* s
       } else {
         Surface sf = surface_at(S, ro, rd, h);
         Frame5& f = F[depth];
-        forward_pass(S, rd, L.lambda, sb, sf, depth, A.P, rng, f);
+        forward_pass(S, rd, L.lambda, sb, sf, depth, A.P, rng, f, stk, stride);
         if (f.ecount != 0u) {  // solveEmission (trace.wesl:91-113)
           f3 lp = sf.pos + f.edir * f.edist;
           f3 so = spawn_origin(sf.pos, sf.gn, f.edir);

@@ -138,6 +138,8 @@ struct Layout {
   std::vector<MatRec> mats;
   std::vector<EmitRec> emit;
   std::vector<float2> alias;
+  std::vector<uint32_t> emit_order;
+  uint32_t emit_root = CG_REF_NONE;
   uint32_t root = CG_REF_NONE;
   uint32_t stack_depth = 0;
 };
@@ -700,11 +702,36 @@ inline int build_sah_accel(const cg_scene_data* sd, Layout& L, std::string& err)
   return CG_OK;
 }
 
+// BVH over the emitters for the light pdf's search (appended pair nodes,
+// leaves index emit_order).  Boxes are padded like the triangle boxes.
+inline void build_emitter_bvh(Layout& L) {
+  if (L.emit.size() < 2) return;
+  std::vector<SahTree::Prim> prims(L.emit.size());
+  float m = 1.0f;
+  for (size_t i = 0; i < L.emit.size(); ++i) {
+    const EmitRec& e = L.emit[i];
+    for (int k = 0; k < 3; ++k) {
+      prims[i].lo[k] = std::min(std::min(e.v0[k], e.v1[k]), e.v2[k]);
+      prims[i].hi[k] = std::max(std::max(e.v0[k], e.v1[k]), e.v2[k]);
+      prims[i].c[k] = 0.5f * (prims[i].lo[k] + prims[i].hi[k]);
+      m = std::max(m, std::max(std::fabs(prims[i].lo[k]), std::fabs(prims[i].hi[k])));
+    }
+  }
+  SahTree t;
+  t.build(prims, CG_LEAF_MAX);
+  L.emit_order = t.order;
+  L.emit_root = emit_pairs(t, L.nodes, 0u, nullptr, 4e-5f * m);
+  L.stack_depth = std::max(L.stack_depth, t.depth + 1u);
+}
+
 // accel 0: the reference's trees; accel 1: rebuilt (SAH, merged world BLAS).
 inline int build_layout(const cg_scene_data* sd, Layout& L, std::string& err, int accel = 1) {
   int rc = build_common(sd, L, err);
   if (rc != CG_OK) return rc;
-  return accel == 0 ? build_reference_accel(sd, L, err) : build_sah_accel(sd, L, err);
+  rc = accel == 0 ? build_reference_accel(sd, L, err) : build_sah_accel(sd, L, err);
+  if (rc != CG_OK) return rc;
+  build_emitter_bvh(L);
+  return CG_OK;
 }
 
 }  // namespace cg
