@@ -85,6 +85,56 @@ def cpu_baseline(scene, sd, conf, cam, budget_s: float, threads: int) -> dict:
                       f"{elapsed:.1f} s), oracle with ordered culling traversal (same hits)"}
 
 
+def traversal_microbench(cg, ev, n_rays: int, n_tris: int, reps: int) -> dict:
+    """The traversal microbenchmark: 2^24 rays (half pinhole rays on a grid,
+    half surface-origin cosine rays) against one 10M-triangle BLAS in 1024
+    Gaussian clusters, closest hit, rays and hits resident in HBM."""
+    t0 = time.perf_counter()
+    scene = cg.HostScene.generate("c2", seed=1, params=[n_tris])
+    sd = scene.scene_data(threads=16)
+    rays = scene.rays(n_rays, seed=1)
+    env = cg.DeviceEnv(cg.Conf.make(64, 64, 1, 1))
+    t1 = time.perf_counter()
+    ds = env.upload(sd)
+    t_upload = time.perf_counter() - t1
+    rb = env.alloc(n_rays * 32).upload(rays)
+    hb = env.alloc(n_rays * 16)
+    setup = time.perf_counter() - t0
+    out = {"workload": f"{n_rays} rays (half coherent pinhole grid, half incoherent surface cosine) vs one "
+                       f"{n_tris}-triangle BLAS (1024 Gaussian clusters)", "setup_s": round(setup, 1),
+           "upload_s": round(t_upload, 1)}
+    half = n_rays // 2
+    stream = env.stream
+    from crystalgraphics_amd import native
+    for name, first, count in (("all", 0, n_rays), ("coherent", 0, half), ("incoherent", half, n_rays - half)):
+        env.set_counting(True)
+        env.reset_stats()
+        native.device().cg_trace_closest(env.handle, ds.handle, rb.ptr + first * 32, count, hb.ptr + first * 16, None)
+        env.sync()
+        st = env.stats()
+        env.set_counting(False)
+        a, b = ev.create(), ev.create()
+        ev.record(a, stream)
+        for _ in range(reps):
+            native.device().cg_trace_closest(env.handle, ds.handle, rb.ptr + first * 32, count, hb.ptr + first * 16,
+                                             None)
+        ev.record(b, stream)
+        env.sync()
+        ms = ev.elapsed_ms(a, b) / reps
+        alg = RAY_IO_BYTES * count + PAIR_BYTES * st["nodes"] + TRI_BYTES * st["tris"] + INST_BYTES * st["insts"]
+        out[name] = {"Mrays/s": round(count / (ms * 1e-3) / 1e6, 1), "ms": round(ms, 3),
+                     "per_query": {"pair_nodes": round(st["nodes"] / count, 2), "triangles": round(st["tris"] / count, 2),
+                                   "instances": round(st["insts"] / count, 2)},
+                     "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1),
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                  "algorithmic_bytes": alg}}
+    rb.free()
+    hb.free()
+    env.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +143,7 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traversal", action="store_true", help="skip the 2^24-ray x 10M-triangle microbenchmark")
     ap.add_argument("--probe-copy-gib", type=float, default=2.0,
                     help="size of the plain-copy HBM probe run before the frames (0: skip)")
     args = ap.parse_args()
@@ -170,6 +221,13 @@ def main() -> int:
             traffic = json.load(f)["traffic_bytes"]
         traffic_src = os.path.relpath(prof, ROOT)
 
+    micro = None
+    if not args.no_traversal:
+        micro = traversal_microbench(cg, ev, 1 << 24, 10_000_000, 3)
+        if probe:
+            for k in ("all", "coherent", "incoherent"):
+                micro[k]["roofline"]["frac_of_measured"] = round(micro[k]["roofline"]["achieved"] / probe["GB/s"], 4)
+
     base = None
     if not args.no_cpu:
         base = cpu_baseline(scene, sd, conf, cam, args.cpu_seconds, min(16, os.cpu_count() or 1))
@@ -210,6 +268,7 @@ def main() -> int:
         },
         "cpu_baseline": base,
         "hbm_probe": probe,
+        "traversal_microbench": micro,
         "latency_ms": None,
     }
     print(json.dumps(out))

@@ -5,7 +5,9 @@
 #define CG_UPLOAD_H_
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <thread>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -368,79 +370,150 @@ struct SahTree {
     return 2.0 * (dx * dy + dy * dz + dz * dx);
   }
 
+  // Subtrees over disjoint ranges of the primitive array are built on
+  // separate threads once a node holds more than kParallel primitives; nodes
+  // come from a pre-sized array through an atomic counter (a node's children
+  // are adjacent).  Primitive records are partitioned in place, so every pass
+  // over a node reads memory sequentially; one pass bins all three axes.
   void build(const std::vector<Prim>& prims, uint32_t max_leaf) {
     const uint32_t n = static_cast<uint32_t>(prims.size());
-    order.resize(n);
-    for (uint32_t i = 0; i < n; ++i) order[i] = i;
     nodes.clear();
-    nodes.reserve(n ? 2 * n : 1);
+    order.clear();
+    depth = 0;
     if (n == 0) return;
-    struct Task {
-      uint32_t node, first, count, depth;
-    };
-    nodes.emplace_back();
-    std::vector<Task> stack{{0u, 0u, n, 0u}};
-    constexpr int kBins = 16;
-    while (!stack.empty()) {
-      Task t = stack.back();
-      stack.pop_back();
-      depth = std::max(depth, t.depth);
+    std::vector<Ref> refs(n);
+    for (uint32_t i = 0; i < n; ++i) refs[i] = Ref{prims[i], i};
+    nodes.resize(2 * static_cast<size_t>(n));
+    std::atomic<uint32_t> next{1u};
+    std::atomic<uint32_t> max_depth{0u};
+    build_rec(refs, max_leaf, 0u, 0u, n, 0u, next, max_depth, 3);
+    nodes.resize(next.load());
+    depth = max_depth.load();
+    order.resize(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = refs[i].id;
+  }
+
+ private:
+  struct Ref {
+    Prim p;
+    uint32_t id;
+  };
+  static constexpr uint32_t kParallel = 1u << 16;
+  static constexpr int kBins = 16;
+  struct Bins {
+    uint32_t cnt[3][kBins];
+    float lo[3][kBins][3], hi[3][kBins][3];
+    void clear() {
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < kBins; ++b) {
+          cnt[a][b] = 0;
+          for (int k = 0; k < 3; ++k) {
+            lo[a][b][k] = INFINITY;
+            hi[a][b][k] = -INFINITY;
+          }
+        }
+    }
+  };
+  void build_rec(std::vector<Ref>& refs, uint32_t max_leaf, uint32_t node, uint32_t first, uint32_t count, uint32_t d,
+                 std::atomic<uint32_t>& next, std::atomic<uint32_t>& max_depth, int par) {
+    for (;;) {
+      uint32_t cur = max_depth.load();
+      while (d > cur && !max_depth.compare_exchange_weak(cur, d)) {
+      }
+      const int chunks = count > (1u << 20) ? 8 : 1;
+      auto for_chunks = [&](auto&& fn) {
+        if (chunks == 1) {
+          fn(0, first, first + count);
+          return;
+        }
+        std::vector<std::thread> pool;
+        for (int c = 0; c < chunks; ++c) {
+          uint32_t b = first + static_cast<uint32_t>(uint64_t(count) * c / chunks);
+          uint32_t e = first + static_cast<uint32_t>(uint64_t(count) * (c + 1) / chunks);
+          pool.emplace_back([&fn, c, b, e] { fn(c, b, e); });
+        }
+        for (auto& t : pool) t.join();
+      };
+      // bounds and centroid bounds
+      float plo[8][3], phi[8][3], pclo[8][3], pchi[8][3];
+      for_chunks([&](int c, uint32_t b, uint32_t e) {
+        for (int k = 0; k < 3; ++k) {
+          plo[c][k] = pclo[c][k] = INFINITY;
+          phi[c][k] = pchi[c][k] = -INFINITY;
+        }
+        for (uint32_t i = b; i < e; ++i) {
+          const Prim& p = refs[i].p;
+          for (int k = 0; k < 3; ++k) {
+            plo[c][k] = std::min(plo[c][k], p.lo[k]);
+            phi[c][k] = std::max(phi[c][k], p.hi[k]);
+            pclo[c][k] = std::min(pclo[c][k], p.c[k]);
+            pchi[c][k] = std::max(pchi[c][k], p.c[k]);
+          }
+        }
+      });
       float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
       float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
-      for (uint32_t i = t.first; i < t.first + t.count; ++i) {
-        const Prim& p = prims[order[i]];
+      for (int c = 0; c < chunks; ++c)
         for (int k = 0; k < 3; ++k) {
-          lo[k] = std::min(lo[k], p.lo[k]);
-          hi[k] = std::max(hi[k], p.hi[k]);
-          clo[k] = std::min(clo[k], p.c[k]);
-          chi[k] = std::max(chi[k], p.c[k]);
+          lo[k] = std::min(lo[k], plo[c][k]);
+          hi[k] = std::max(hi[k], phi[c][k]);
+          clo[k] = std::min(clo[k], pclo[c][k]);
+          chi[k] = std::max(chi[k], pchi[c][k]);
         }
-      }
-      Node& nd = nodes[t.node];
+      Node& nd = nodes[node];
       for (int k = 0; k < 3; ++k) {
         nd.lo[k] = lo[k];
         nd.hi[k] = hi[k];
       }
-      auto make_leaf = [&]() {
-        nodes[t.node].first = t.first;
-        nodes[t.node].count = t.count;
-      };
-      if (t.count <= 1) {
-        make_leaf();
-        continue;
+      if (count <= 1) {
+        nd.first = first;
+        nd.count = count;
+        return;
       }
-      // best split over three axes x 15 planes
-      double best_cost = INFINITY;
-      int best_axis = -1, best_plane = 0;
-      for (int axis = 0; axis < 3; ++axis) {
-        const float ext = chi[axis] - clo[axis];
-        if (!(ext > 0.0f)) continue;
-        const double scale = kBins / double(ext);
-        uint32_t cnt[kBins] = {};
-        float blo[kBins][3], bhi[kBins][3];
-        for (int b = 0; b < kBins; ++b)
-          for (int k = 0; k < 3; ++k) {
-            blo[b][k] = INFINITY;
-            bhi[b][k] = -INFINITY;
-          }
-        for (uint32_t i = t.first; i < t.first + t.count; ++i) {
-          const Prim& p = prims[order[i]];
-          int b = std::min(kBins - 1, static_cast<int>((p.c[axis] - clo[axis]) * scale));
-          cnt[b]++;
-          for (int k = 0; k < 3; ++k) {
-            blo[b][k] = std::min(blo[b][k], p.lo[k]);
-            bhi[b][k] = std::max(bhi[b][k], p.hi[k]);
+      // bin all three axes in one pass
+      double scale[3];
+      for (int a = 0; a < 3; ++a) scale[a] = chi[a] > clo[a] ? kBins / double(chi[a] - clo[a]) : 0.0;
+      Bins local;
+      std::vector<Bins> heap(chunks > 1 ? chunks : 0);
+      Bins* part = chunks > 1 ? heap.data() : &local;
+      for_chunks([&](int c, uint32_t b0, uint32_t e0) {
+        Bins& B = part[c];
+        B.clear();
+        for (uint32_t i = b0; i < e0; ++i) {
+          const Prim& p = refs[i].p;
+          for (int a = 0; a < 3; ++a) {
+            int b = std::min(kBins - 1, static_cast<int>((p.c[a] - clo[a]) * scale[a]));
+            B.cnt[a][b]++;
+            for (int k = 0; k < 3; ++k) {
+              B.lo[a][b][k] = std::min(B.lo[a][b][k], p.lo[k]);
+              B.hi[a][b][k] = std::max(B.hi[a][b][k], p.hi[k]);
+            }
           }
         }
+      });
+      Bins& B = part[0];
+      for (int c = 1; c < chunks; ++c)
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < kBins; ++b) {
+            B.cnt[a][b] += part[c].cnt[a][b];
+            for (int k = 0; k < 3; ++k) {
+              B.lo[a][b][k] = std::min(B.lo[a][b][k], part[c].lo[a][b][k]);
+              B.hi[a][b][k] = std::max(B.hi[a][b][k], part[c].hi[a][b][k]);
+            }
+          }
+      double best_cost = INFINITY;
+      int best_axis = -1, best_plane = 0;
+      for (int a = 0; a < 3; ++a) {
+        if (scale[a] == 0.0) continue;
         double right_area[kBins];
         uint32_t right_cnt[kBins];
         float rlo[3] = {INFINITY, INFINITY, INFINITY}, rhi[3] = {-INFINITY, -INFINITY, -INFINITY};
         uint32_t rc = 0;
         for (int b = kBins - 1; b > 0; --b) {
-          rc += cnt[b];
+          rc += B.cnt[a][b];
           for (int k = 0; k < 3; ++k) {
-            rlo[k] = std::min(rlo[k], blo[b][k]);
-            rhi[k] = std::max(rhi[k], bhi[b][k]);
+            rlo[k] = std::min(rlo[k], B.lo[a][b][k]);
+            rhi[k] = std::max(rhi[k], B.hi[a][b][k]);
           }
           right_area[b] = area(rlo, rhi);
           right_cnt[b] = rc;
@@ -448,47 +521,57 @@ struct SahTree {
         float llo[3] = {INFINITY, INFINITY, INFINITY}, lhi[3] = {-INFINITY, -INFINITY, -INFINITY};
         uint32_t lc = 0;
         for (int b = 0; b < kBins - 1; ++b) {
-          lc += cnt[b];
+          lc += B.cnt[a][b];
           for (int k = 0; k < 3; ++k) {
-            llo[k] = std::min(llo[k], blo[b][k]);
-            lhi[k] = std::max(lhi[k], bhi[b][k]);
+            llo[k] = std::min(llo[k], B.lo[a][b][k]);
+            lhi[k] = std::max(lhi[k], B.hi[a][b][k]);
           }
           if (lc == 0 || right_cnt[b + 1] == 0) continue;
-          double c = area(llo, lhi) * lc + right_area[b + 1] * right_cnt[b + 1];
-          if (c < best_cost) {
-            best_cost = c;
-            best_axis = axis;
+          double cst = area(llo, lhi) * lc + right_area[b + 1] * right_cnt[b + 1];
+          if (cst < best_cost) {
+            best_cost = cst;
+            best_axis = a;
             best_plane = b;
           }
         }
       }
       const double parent_area = std::max(area(lo, hi), 1e-30);
       const double split_cost = 1.0 + best_cost / parent_area;  // traversal 1, intersection 1 per primitive
-      if (t.count <= max_leaf && (best_axis < 0 || double(t.count) <= split_cost)) {
-        make_leaf();
-        continue;
+      if (count <= max_leaf && (best_axis < 0 || double(count) <= split_cost)) {
+        nd.first = first;
+        nd.count = count;
+        return;
       }
       uint32_t mid;
       if (best_axis < 0) {  // all centroids equal: split by count
-        mid = t.first + t.count / 2;
+        mid = first + count / 2;
       } else {
-        const double scale = kBins / double(chi[best_axis] - clo[best_axis]);
+        const double sc = scale[best_axis];
         const float c0 = clo[best_axis];
-        auto it = std::partition(order.begin() + t.first, order.begin() + t.first + t.count, [&](uint32_t id) {
-          int b = std::min(kBins - 1, static_cast<int>((prims[id].c[best_axis] - c0) * scale));
+        auto it = std::partition(refs.begin() + first, refs.begin() + first + count, [&](const Ref& r) {
+          int b = std::min(kBins - 1, static_cast<int>((r.p.c[best_axis] - c0) * sc));
           return b <= best_plane;
         });
-        mid = static_cast<uint32_t>(it - order.begin());
-        if (mid == t.first || mid == t.first + t.count) mid = t.first + t.count / 2;
+        mid = static_cast<uint32_t>(it - refs.begin());
+        if (mid == first || mid == first + count) mid = first + count / 2;
       }
-      const uint32_t left = static_cast<uint32_t>(nodes.size());
-      nodes.emplace_back();
-      nodes.emplace_back();
-      nodes[t.node].left = left;
-      nodes[t.node].right = left + 1;
-      // right pushed first so the left subtree is processed (and numbered) first
-      stack.push_back({left + 1, mid, t.first + t.count - mid, t.depth + 1});
-      stack.push_back({left, t.first, mid - t.first, t.depth + 1});
+      const uint32_t left = next.fetch_add(2u);
+      nodes[node].left = left;
+      nodes[node].right = left + 1;
+      const uint32_t lcount = mid - first, rcount = first + count - mid;
+      if (par > 0 && count > kParallel) {
+        std::thread t([&, left, first, lcount, d, par] {
+          build_rec(refs, max_leaf, left, first, lcount, d + 1, next, max_depth, par - 1);
+        });
+        build_rec(refs, max_leaf, left + 1, mid, rcount, d + 1, next, max_depth, par - 1);
+        t.join();
+        return;
+      }
+      build_rec(refs, max_leaf, left, first, lcount, d + 1, next, max_depth, 0);
+      node = left + 1;  // continue with the right child in this frame
+      first = mid;
+      count = rcount;
+      ++d;
     }
   }
 };

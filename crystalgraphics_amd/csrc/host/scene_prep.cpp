@@ -100,19 +100,23 @@ struct BlasBuilder {
   V3 pos(uint32_t t, int k) const { return ld(verts[idx[prim.index_offset + t * 3 + k]].pos); }
 
   // node: local index (0 = root); next: first local index of its descendants.
-  void build(uint32_t node, uint32_t l, uint32_t r, uint32_t next, int par_depth) {
-    V3 lb{FLT_MAX, FLT_MAX, FLT_MAX}, ub{-FLT_MAX, -FLT_MAX, -FLT_MAX};
-    for (uint32_t i = l; i < r; ++i) {
-      V3 v0 = pos(info[i].index, 0), v1 = pos(info[i].index, 1), v2 = pos(info[i].index, 2);
-      lb = vmin(lb, vmin(vmin(v0, v1), v2));
-      ub = vmax(ub, vmax(vmax(v0, v1), v2));
-    }
-    V3 pad{0.001f, 0.001f, 0.001f};
-    cg_blas_node& n = nodes[node];
-    st(n.lb, sub(lb, pad));
-    st(n.ub, add(ub, pad));
+  // Returns the node's unpadded box.  Boxes are unions of exact minima and
+  // maxima, so building them bottom-up gives the same floats as scanning every
+  // triangle of the node (blas.h:187-201) at O(n) instead of O(n log n).
+  void build(uint32_t node, uint32_t l, uint32_t r, uint32_t next, int par_depth, V3& lb, V3& ub) {
     const uint32_t count = r - l;
+    const V3 pad{0.001f, 0.001f, 0.001f};
+    cg_blas_node& n = nodes[node];
     if (count <= 2) {
+      lb = V3{FLT_MAX, FLT_MAX, FLT_MAX};
+      ub = V3{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+      for (uint32_t i = l; i < r; ++i) {
+        V3 v0 = pos(info[i].index, 0), v1 = pos(info[i].index, 1), v2 = pos(info[i].index, 2);
+        lb = vmin(lb, vmin(vmin(v0, v1), v2));
+        ub = vmax(ub, vmax(vmax(v0, v1), v2));
+      }
+      st(n.lb, sub(lb, pad));
+      st(n.ub, add(ub, pad));
       n.child = tri_offset + l;
       n.triangle_count = count;
       return;
@@ -131,14 +135,19 @@ struct BlasBuilder {
     n.triangle_count = 0;
     const uint32_t left_next = next + 2;
     const uint32_t right_next = next + 2 + static_cast<uint32_t>(2 * interior_nodes(mid - l));
+    V3 llb, lub, rlb, rub;
     if (par_depth > 0 && count > 65536) {
-      std::thread t([&] { build(c, l, mid, left_next, par_depth - 1); });
-      build(c + 1, mid, r, right_next, par_depth - 1);
+      std::thread t([&] { build(c, l, mid, left_next, par_depth - 1, llb, lub); });
+      build(c + 1, mid, r, right_next, par_depth - 1, rlb, rub);
       t.join();
     } else {
-      build(c, l, mid, left_next, 0);
-      build(c + 1, mid, r, right_next, 0);
+      build(c, l, mid, left_next, 0, llb, lub);
+      build(c + 1, mid, r, right_next, 0, rlb, rub);
     }
+    lb = vmin(llb, rlb);
+    ub = vmax(lub, rub);
+    st(nodes[node].lb, sub(lb, pad));
+    st(nodes[node].ub, add(ub, pad));
   }
 };
 
@@ -241,7 +250,8 @@ SceneData BuildSceneData(const Scene::Impl& sc, bool share_blas, int threads) {
     init.ub[0] = init.ub[1] = init.ub[2] = FLT_MIN;
     sd.blas.resize(base + nodes, init);
     BlasBuilder b{pr, verts, idx, info, sd.blas.data() + base, base, offset};
-    b.build(0, 0, tcount, 1, par_depth);
+    V3 rlb, rub;
+    b.build(0, 0, tcount, 1, par_depth, rlb, rub);
     for (const TriInfo& t : info) {
       const uint32_t o = pr.index_offset + t.index * 3;
       sd.indices.push_back(cg_index{idx[o], idx[o + 1], idx[o + 2], 0u});
