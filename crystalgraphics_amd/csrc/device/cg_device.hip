@@ -44,7 +44,8 @@ struct cg_env {
   hipStream_t stream = nullptr;
   std::string last_error;
   DevBuf history, rgba, path_hash, ray_count, stats;
-  DevBuf wl_cdf, wl_leaf;
+  DevBuf wl_cdf, wl_leaf, queue;
+  int cu_count = 0;
   float wl_avg = 0.0f;
   bool counting = false;
   bool path_records = false;
@@ -91,12 +92,47 @@ int dev_upload(cg_env* env, DevBuf& b, const void* src, size_t bytes) {
   return CG_OK;
 }
 
-// Blocks are issued round-robin over the 8 XCDs; remap so each XCD works on
-// one contiguous range of tiles and its L2 sees neighbouring rays.
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t bid, uint32_t nb) {
-  const uint32_t kXcd = 8u;
-  uint32_t per = nb / kXcd, rem = nb % kXcd, x = bid % kXcd, local = bid / kXcd;
-  return x < rem ? x * (per + 1u) + local : rem * (per + 1u) + (x - rem) * per + local;
+// Work distribution.  Blocks are issued round-robin over the 8 XCDs
+// (block b runs on XCD b % 8).  Work items are split into 8 contiguous
+// ranges, one per XCD, so an XCD's L2 sees neighbouring rays or pixels; each
+// block takes items from its own XCD's range through an atomic counter and,
+// once that range is exhausted, steals from the next XCD's.  The grid is the
+// resident size, so blocks loop until every range is empty.
+constexpr uint32_t kXcds = 8u;
+constexpr uint32_t kNoItem = 0xFFFFFFFFu;
+struct WorkQueue {
+  uint32_t* counters;  // kXcds counters, zero at launch
+  uint32_t total;      // items
+};
+__device__ __forceinline__ uint32_t grab_item(const WorkQueue& q, uint32_t& probe, uint32_t* s_item) {
+  if (threadIdx.x == 0) {
+    const uint32_t per = (q.total + kXcds - 1u) / kXcds;
+    const uint32_t home = blockIdx.x % kXcds;
+    uint32_t item = kNoItem;
+    while (probe < kXcds) {
+      const uint32_t x = (home + probe) % kXcds;
+      const uint32_t begin = x * per;
+      const uint32_t end = begin + per < q.total ? begin + per : q.total;
+      if (begin < end) {
+        const uint32_t c = atomicAdd(q.counters + x, 1u);
+        if (begin + c < end) {
+          item = begin + c;
+          break;
+        }
+      }
+      ++probe;
+    }
+    *s_item = item;
+  }
+  __syncthreads();
+  const uint32_t item = *s_item;
+  __syncthreads();
+  return item;
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
 __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
@@ -129,12 +165,18 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const cg_ray* __re
                                                   const float* __restrict__ tmax, uint32_t n,
                                                   cg_hit* __restrict__ hits, float* __restrict__ uv,
                                                   uint32_t* __restrict__ occluded,
-                                                  unsigned long long* __restrict__ stats) {
+                                                  unsigned long long* __restrict__ stats, WorkQueue q) {
   __shared__ uint32_t stk[SS * kBlock];
-  uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  uint32_t i = tile * kBlock + threadIdx.x;
+  __shared__ uint32_t s_item;
   TravStats st{0u, 0u, 0u};
-  if (i < n) {
+  uint64_t nq = 0;
+  uint32_t probe = 0;
+  for (;;) {
+    const uint32_t item = grab_item(q, probe, &s_item);
+    if (item == kNoItem) break;
+    const uint32_t i = item * kBlock + threadIdx.x;
+    if (i >= n) continue;
+    ++nq;
     const float4* rp = reinterpret_cast<const float4*>(rays + i);
     float4 a = rp[0], b = rp[1];
     f3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z);
@@ -161,8 +203,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const cg_ray* __re
       }
     }
   }
-  if (stats) flush_stats(stats, (i < n && !kAnyHit) ? 1u : 0u, (i < n && kAnyHit) ? 1u : 0u, st.nodes, st.tris,
-                         st.insts);
+  if (stats) flush_stats(stats, kAnyHit ? 0u : nq, kAnyHit ? nq : 0u, st.nodes, st.tris, st.insts);
 }
 
 // ---------------------------------------------------------------------------
@@ -171,15 +212,20 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const cg_ray* __re
 // the reference's depth-first order.  The transport tree's open frames live
 // in a private array (MF frames = trace_depth - 1); the BVH stack in LDS.
 template <int SS, int MF>
-__global__ __launch_bounds__(kBlock, 4) void k_view(ViewArgs A) {
+__global__ __launch_bounds__(kBlock, 4) void k_view(ViewArgs A, WorkQueue q) {
   __shared__ uint32_t stk_lds[SS * kBlock];
+  __shared__ uint32_t s_item;
   const uint32_t tiles_x = (A.W + 15u) / 16u;
-  uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  uint32_t px = (tile % tiles_x) * 16u + (threadIdx.x & 15u);
-  uint32_t py = (tile / tiles_x) * 16u + (threadIdx.x >> 4);
   TravStats st{0u, 0u, 0u};
   uint64_t nrays = 0, nshadow = 0;
-  if (px < A.W && py < A.H) render_pixel<MF>(A, px, py, stk_lds + threadIdx.x, kBlock, st, nrays, nshadow);
+  uint32_t probe = 0;
+  for (;;) {
+    const uint32_t tile = grab_item(q, probe, &s_item);
+    if (tile == kNoItem) break;
+    const uint32_t px = (tile % tiles_x) * 16u + (threadIdx.x & 15u);
+    const uint32_t py = (tile / tiles_x) * 16u + (threadIdx.x >> 4);
+    if (px < A.W && py < A.H) render_pixel<MF>(A, px, py, stk_lds + threadIdx.x, kBlock, st, nrays, nshadow);
+  }
   if (A.stats) flush_stats(A.stats, nrays, nshadow, st.nodes, st.tris, st.insts);
 }
 
@@ -209,20 +255,44 @@ void release_scene(cg_scene* s) {
   dev_free(s->env);
 }
 
+// Resident grid of a kernel: blocks per CU at full occupancy x CUs, capped by
+// the number of work items.
+template <typename K>
+uint32_t resident_grid(K kernel, const cg_env* env, uint32_t items) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  const uint64_t g = static_cast<uint64_t>(per_cu) * static_cast<uint64_t>(env->cu_count > 0 ? env->cu_count : 1);
+  return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(g, items)));
+}
+WorkQueue reset_queue(cg_env* env, uint32_t items) {
+  uint32_t* c = static_cast<uint32_t*>(env->queue.p);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kXcds), 0, env->stream, c, kXcds);
+  return WorkQueue{c, items};
+}
 template <int SS, int MF>
-void launch_view_mf(const ViewArgs& a, dim3 grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_view<SS, MF>), grid, dim3(kBlock), 0, s, a);
+void launch_view_mf(cg_env* env, const ViewArgs& a, uint32_t tiles) {
+  WorkQueue q = reset_queue(env, tiles);
+  hipLaunchKernelGGL((k_view<SS, MF>), dim3(resident_grid(k_view<SS, MF>, env, tiles)), dim3(kBlock), 0, env->stream, a,
+                     q);
 }
 template <int SS>
-void launch_view_ss(const ViewArgs& a, int frames, dim3 grid, hipStream_t s) {
+void launch_view_ss(cg_env* env, const ViewArgs& a, int frames, uint32_t tiles) {
   if (frames <= 4)
-    launch_view_mf<SS, 4>(a, grid, s);
+    launch_view_mf<SS, 4>(env, a, tiles);
   else if (frames <= 8)
-    launch_view_mf<SS, 8>(a, grid, s);
+    launch_view_mf<SS, 8>(env, a, tiles);
   else if (frames <= 16)
-    launch_view_mf<SS, 16>(a, grid, s);
+    launch_view_mf<SS, 16>(env, a, tiles);
   else
-    launch_view_mf<SS, 64>(a, grid, s);
+    launch_view_mf<SS, 64>(env, a, tiles);
+}
+template <int SS, bool kAnyHit>
+void launch_trace(cg_env* env, const DevScene& S, const cg_ray* rays, const float* tmax, uint32_t n, cg_hit* hits,
+                  float* uv, uint32_t* occ, unsigned long long* st) {
+  const uint32_t items = (n + kBlock - 1) / kBlock;
+  WorkQueue q = reset_queue(env, items);
+  hipLaunchKernelGGL((k_trace<SS, kAnyHit>), dim3(resident_grid(k_trace<SS, kAnyHit>, env, items)), dim3(kBlock), 0,
+                     env->stream, S, rays, tmax, n, hits, uv, occ, st, q);
 }
 
 }  // namespace
@@ -261,6 +331,10 @@ int cg_env_create(const cg_conf* conf, int device, cg_env** out) {
   if ((rc = dev_alloc(env, env->history, px * sizeof(cg_history_pixel))) != CG_OK) return bail(rc);
   if ((rc = dev_alloc(env, env->rgba, px * 4)) != CG_OK) return bail(rc);
   if ((rc = dev_alloc(env, env->stats, 8 * sizeof(unsigned long long))) != CG_OK) return bail(rc);
+  if ((rc = dev_alloc(env, env->queue, kXcds * sizeof(uint32_t))) != CG_OK) return bail(rc);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return bail(fail(env, CG_ERR_HIP, "hipGetDeviceProperties failed"));
+  env->cu_count = prop.multiProcessorCount;
   if (hipMemsetAsync(env->history.p, 0, env->history.bytes, env->stream) != hipSuccess ||
       hipMemsetAsync(env->stats.p, 0, env->stats.bytes, env->stream) != hipSuccess)
     return bail(fail(env, CG_ERR_HIP, "hipMemsetAsync failed"));
@@ -290,6 +364,7 @@ int cg_env_destroy(cg_env* env) {
   dev_free(env->stats);
   dev_free(env->wl_cdf);
   dev_free(env->wl_leaf);
+  dev_free(env->queue);
   if (env->stream) (void)hipStreamDestroy(env->stream);
   delete env;
   return CG_OK;
@@ -454,9 +529,9 @@ int cg_view(cg_env* env, cg_scene* scene, const cg_camera* camera, uint32_t iter
   const uint32_t tiles = ((a.W + 15u) / 16u) * ((a.H + 15u) / 16u);
   const int frames = std::max(1, a.P.D - 1);
   if (scene->stack_depth <= 32u)
-    launch_view_ss<32>(a, frames, dim3(tiles), env->stream);
+    launch_view_ss<32>(env, a, frames, tiles);
   else
-    launch_view_ss<64>(a, frames, dim3(tiles), env->stream);
+    launch_view_ss<64>(env, a, frames, tiles);
   CG_HIP(env, hipGetLastError());
   return CG_OK;
 }
@@ -490,14 +565,11 @@ int cg_write_history(cg_env* env, const cg_history_pixel* history) {
 int cg_trace_closest(cg_env* env, cg_scene* scene, const cg_ray* rays, uint32_t n, cg_hit* hits, float* uv) {
   if (!env || !scene || (!rays && n) || (!hits && n)) return fail(env, CG_ERR_INVALID, "cg_trace_closest: null argument");
   if (n == 0) return CG_OK;
-  dim3 grid((n + kBlock - 1) / kBlock);
   unsigned long long* st = env->counting ? static_cast<unsigned long long*>(env->stats.p) : nullptr;
   if (scene->stack_depth <= 32u)
-    hipLaunchKernelGGL((k_trace<32, false>), grid, dim3(kBlock), 0, env->stream, scene->view, rays, nullptr, n, hits,
-                       uv, nullptr, st);
+    launch_trace<32, false>(env, scene->view, rays, nullptr, n, hits, uv, nullptr, st);
   else
-    hipLaunchKernelGGL((k_trace<64, false>), grid, dim3(kBlock), 0, env->stream, scene->view, rays, nullptr, n, hits,
-                       uv, nullptr, st);
+    launch_trace<64, false>(env, scene->view, rays, nullptr, n, hits, uv, nullptr, st);
   CG_HIP(env, hipGetLastError());
   return CG_OK;
 }
@@ -507,14 +579,11 @@ int cg_trace_occluded(cg_env* env, cg_scene* scene, const cg_ray* rays, const fl
   if (!env || !scene || (n && (!rays || !tmax || !occluded)))
     return fail(env, CG_ERR_INVALID, "cg_trace_occluded: null argument");
   if (n == 0) return CG_OK;
-  dim3 grid((n + kBlock - 1) / kBlock);
   unsigned long long* st = env->counting ? static_cast<unsigned long long*>(env->stats.p) : nullptr;
   if (scene->stack_depth <= 32u)
-    hipLaunchKernelGGL((k_trace<32, true>), grid, dim3(kBlock), 0, env->stream, scene->view, rays, tmax, n, nullptr,
-                       nullptr, occluded, st);
+    launch_trace<32, true>(env, scene->view, rays, tmax, n, nullptr, nullptr, occluded, st);
   else
-    hipLaunchKernelGGL((k_trace<64, true>), grid, dim3(kBlock), 0, env->stream, scene->view, rays, tmax, n, nullptr,
-                       nullptr, occluded, st);
+    launch_trace<64, true>(env, scene->view, rays, tmax, n, nullptr, nullptr, occluded, st);
   CG_HIP(env, hipGetLastError());
   return CG_OK;
 }
