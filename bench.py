@@ -27,10 +27,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of one traversal query in the device layout: ray in (32)
-# + hit out (16) + 64 B per pair node fetched + 48 B per triangle record
-# tested + 80 B per instance record entered.
+# + hit out (16) + 128 B per 4-wide BVH node visited + 48 B per triangle
+# record tested + 80 B per instance record entered.
 RAY_IO_BYTES = 32 + 16
-PAIR_BYTES, TRI_BYTES, INST_BYTES = 64, 48, 80
+NODE_BYTES, TRI_BYTES, INST_BYTES = 128, 48, 80
 
 
 class HipEvents:
@@ -121,9 +121,9 @@ def traversal_microbench(cg, ev, n_rays: int, n_tris: int, reps: int) -> dict:
         ev.record(b, stream)
         env.sync()
         ms = ev.elapsed_ms(a, b) / reps
-        alg = RAY_IO_BYTES * count + PAIR_BYTES * st["nodes"] + TRI_BYTES * st["tris"] + INST_BYTES * st["insts"]
+        alg = RAY_IO_BYTES * count + NODE_BYTES * st["nodes"] + TRI_BYTES * st["tris"] + INST_BYTES * st["insts"]
         out[name] = {"Mrays/s": round(count / (ms * 1e-3) / 1e6, 1), "ms": round(ms, 3),
-                     "per_query": {"pair_nodes": round(st["nodes"] / count, 2), "triangles": round(st["tris"] / count, 2),
+                     "per_query": {"nodes": round(st["nodes"] / count, 2), "triangles": round(st["tris"] / count, 2),
                                    "instances": round(st["insts"] / count, 2)},
                      "roofline": {"bound": "hbm", "achieved": round(alg / (ms * 1e-3) / 1e9, 1),
                                   "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -144,6 +144,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traversal", action="store_true", help="skip the 2^24-ray x 10M-triangle microbenchmark")
+    ap.add_argument("--traversal-reps", type=int, default=3, help="timed launches per microbenchmark batch")
     ap.add_argument("--probe-copy-gib", type=float, default=2.0,
                     help="size of the plain-copy HBM probe run before the frames (0: skip)")
     args = ap.parse_args()
@@ -209,7 +210,7 @@ def main() -> int:
     ms = ms_total / args.steps
     mrays = rays / (ms * 1e-3) / 1e6
 
-    alg_bytes = RAY_IO_BYTES * rays + PAIR_BYTES * st["nodes"] + TRI_BYTES * st["tris"] + INST_BYTES * st["insts"]
+    alg_bytes = RAY_IO_BYTES * rays + NODE_BYTES * st["nodes"] + TRI_BYTES * st["tris"] + INST_BYTES * st["insts"]
     achieved = alg_bytes / (ms * 1e-3) / 1e9
 
     # HBM bytes per launch come from the committed rocprofv3 --pmc runs of this
@@ -223,7 +224,7 @@ def main() -> int:
 
     micro = None
     if not args.no_traversal:
-        micro = traversal_microbench(cg, ev, 1 << 24, 10_000_000, 3)
+        micro = traversal_microbench(cg, ev, 1 << 24, 10_000_000, args.traversal_reps)
         if probe:
             for k in ("all", "coherent", "incoherent"):
                 micro[k]["roofline"]["frac_of_measured"] = round(micro[k]["roofline"]["achieved"] / probe["GB/s"], 4)
@@ -263,7 +264,7 @@ def main() -> int:
             "traffic": None if traffic is None else round(traffic),
             "traffic_source": traffic_src,
             "kernel": "k_view",
-            "per_launch": {"pair_nodes": st["nodes"], "triangles": st["tris"], "instances": st["insts"],
+            "per_launch": {"nodes": st["nodes"], "triangles": st["tris"], "instances": st["insts"],
                            "queries": rays, "algorithmic_bytes": alg_bytes},
         },
         "cpu_baseline": base,

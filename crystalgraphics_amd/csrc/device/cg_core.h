@@ -162,7 +162,8 @@ CG_HD TriT tri_test(f3 o, f3 d, f3 v0, f3 e1, f3 e2) {
 // ---------------------------------------------------------------------------
 // Scene view used by the kernels.
 struct DevScene {
-  const PairNode* nodes;
+  const Node4* bvh;        // scene BVH, 4-wide
+  const PairNode* nodes;   // emitter BVH
   const TriRec* tris;
   const InstRec* insts;
   const uint4* tri_vidx;   // per slot: source vertex indices
@@ -189,42 +190,103 @@ struct HitRec {
   uint32_t inst, slot;
 };
 
-// Two-level closest-hit (or any-hit) traversal over 64-byte pair nodes:
-// near child first, far child pushed, boxes entered beyond the best hit so far
-// skipped, a marker on the stack to return from a BLAS to the TLAS.  The
-// closest hit equals the reference's unordered walk (bvh.wesl:56-186) except
-// between triangles hit at exactly the same distance.  `stk` holds `stride`-
-// interleaved entries (LDS: stride = workgroup size).
+// Traversal stack: the first kShortStack entries of each lane live in
+// `ref`/`t` (LDS, `stride`-interleaved), deeper entries in `ovf` (device
+// memory, `ovf_stride`-interleaved; sized from the tree's stack bound).  An
+// entry is a node ref and the distance at which the ray enters that node.
+constexpr int kShortStack = 16;
+struct TStack {
+  uint32_t* ref;
+  float* t;
+  int stride;
+  uint2* ovf;
+  uint32_t ovf_stride;
+  int ovf_cap;  // overflow entries per lane (the bound says none are needed beyond)
+  CG_HD void push(int& sp, uint32_t r, float tt) const {
+    if (sp < kShortStack) {
+      ref[sp * stride] = r;
+      t[sp * stride] = tt;
+    } else if (sp - kShortStack < ovf_cap) {
+      ovf[static_cast<size_t>(sp - kShortStack) * ovf_stride] = make_uint2(r, __float_as_uint(tt));
+    }
+    ++sp;
+  }
+  CG_HD void pop(int& sp, uint32_t& r, float& tt) const {
+    --sp;
+    if (sp < kShortStack) {
+      r = ref[sp * stride];
+      tt = t[sp * stride];
+    } else if (sp - kShortStack < ovf_cap) {
+      const uint2 e = ovf[static_cast<size_t>(sp - kShortStack) * ovf_stride];
+      r = e.x;
+      tt = __uint_as_float(e.y);
+    } else {
+      r = CG_REF_NONE;
+      tt = __builtin_inff();
+    }
+  }
+};
+
+CG_HD void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
+  if (tb < ta) {
+    float t = ta;
+    ta = tb;
+    tb = t;
+    uint32_t r = ra;
+    ra = rb;
+    rb = r;
+  }
+}
+
+// Two-level closest-hit (or any-hit) traversal of the 4-wide BVH: the hit
+// children of a node sorted by entry distance, the nearest visited next and
+// the others pushed with their distances; popped entries entered beyond the
+// best hit so far are dropped; a marker on the stack returns from a BLAS to
+// the TLAS.  The closest hit equals the reference's unordered walk
+// (bvh.wesl:56-186) except between triangles hit at exactly the same
+// distance.
 template <bool kAnyHit>
-CG_HD HitRec traverse(const DevScene& S, f3 wo, f3 wd, float tmax, uint32_t* stk, int stride,
-                      TravStats& st) {
+CG_HD HitRec traverse(const DevScene& S, f3 wo, f3 wd, float tmax, const TStack& stk, TravStats& st) {
   HitRec best{tmax, 0.0f, 0.0f, CG_MISS_ID, CG_MISS_ID};
   uint32_t ref = S.root;
   if (ref == CG_REF_NONE) return best;
+  const float kInf = __builtin_inff();
   const f3 winv = dir_inv(wd);
   f3 o = wo, d = wd, inv = winv;
   uint32_t cur_inst = CG_MISS_ID;
   int sp = 0;
   for (;;) {
     if ((ref & (CG_REF_LEAF | CG_REF_INST)) == 0u) {
-      const PairNode* n = S.nodes + ref;
+      const float4* q = reinterpret_cast<const float4*>(S.bvh + ref);
+      const float4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
+      const uint4 rr = reinterpret_cast<const uint4*>(q)[6];
       st.nodes++;
-      float tl = box_enter(o, inv, n->lo_l, n->hi_l);
-      float tr = box_enter(o, inv, n->lo_r, n->hi_r);
-      bool hl = tl <= best.t, hr = tr <= best.t;
-      if (hl && hr) {
-        bool left_first = tl <= tr;
-        stk[sp * stride] = left_first ? n->ref_r : n->ref_l;
-        ++sp;
-        ref = left_first ? n->ref_l : n->ref_r;
-        continue;
+      float t0, t1, t2, t3;
+      uint32_t r0 = rr.x, r1 = rr.y, r2 = rr.z, r3 = rr.w;
+      {
+        const float b0[3] = {lx.x, ly.x, lz.x}, e0[3] = {hx.x, hy.x, hz.x};
+        const float b1[3] = {lx.y, ly.y, lz.y}, e1[3] = {hx.y, hy.y, hz.y};
+        const float b2[3] = {lx.z, ly.z, lz.z}, e2[3] = {hx.z, hy.z, hz.z};
+        const float b3[3] = {lx.w, ly.w, lz.w}, e3[3] = {hx.w, hy.w, hz.w};
+        t0 = box_enter(o, inv, b0, e0);
+        t1 = box_enter(o, inv, b1, e1);
+        t2 = box_enter(o, inv, b2, e2);
+        t3 = box_enter(o, inv, b3, e3);
       }
-      if (hl) {
-        ref = n->ref_l;
-        continue;
-      }
-      if (hr) {
-        ref = n->ref_r;
+      t0 = (t0 <= best.t && r0 != CG_REF_NONE) ? t0 : kInf;
+      t1 = (t1 <= best.t && r1 != CG_REF_NONE) ? t1 : kInf;
+      t2 = (t2 <= best.t && r2 != CG_REF_NONE) ? t2 : kInf;
+      t3 = (t3 <= best.t && r3 != CG_REF_NONE) ? t3 : kInf;
+      cswap(t0, r0, t1, r1);
+      cswap(t2, r2, t3, r3);
+      cswap(t0, r0, t2, r2);
+      cswap(t1, r1, t3, r3);
+      cswap(t1, r1, t2, r2);
+      if (t3 < kInf) stk.push(sp, r3, t3);
+      if (t2 < kInf) stk.push(sp, r2, t2);
+      if (t1 < kInf) stk.push(sp, r1, t1);
+      if (t0 < kInf) {
+        ref = r0;
         continue;
       }
     } else if ((ref & CG_REF_LEAF) == 0u) {
@@ -236,8 +298,7 @@ CG_HD HitRec traverse(const DevScene& S, f3 wo, f3 wd, float tmax, uint32_t* stk
         d = mat_dir(in->inv, wd);
         inv = dir_inv(d);
         cur_inst = ii;
-        stk[sp * stride] = CG_REF_MARKER;
-        ++sp;
+        stk.push(sp, CG_REF_MARKER, 0.0f);
         ref = in->root;
         continue;
       }
@@ -261,13 +322,16 @@ CG_HD HitRec traverse(const DevScene& S, f3 wo, f3 wd, float tmax, uint32_t* stk
     // pop
     for (;;) {
       if (sp == 0) return best;
-      --sp;
-      ref = stk[sp * stride];
-      if (ref != CG_REF_MARKER) break;
-      o = wo;
-      d = wd;
-      inv = winv;
-      cur_inst = CG_MISS_ID;
+      float te;
+      stk.pop(sp, ref, te);
+      if (ref == CG_REF_MARKER) {
+        o = wo;
+        d = wd;
+        inv = winv;
+        cur_inst = CG_MISS_ID;
+        continue;
+      }
+      if (te <= best.t) break;
     }
   }
 }
@@ -692,7 +756,7 @@ CG_HD float light_pdf_linear(const DevScene& S, f3 rd, f3 pos) {
 // the emitters (padded boxes, no distance cut-off) and added in ascending
 // emitter index, so the result equals the linear loop's bit for bit.  More
 // than kCap hits fall back to the linear loop.
-CG_HD float light_pdf(const DevScene& S, f3 next, f3 pos, uint32_t* stk, int stride) {
+CG_HD float light_pdf(const DevScene& S, f3 next, f3 pos, const TStack& stk) {
   if (S.alias_count == 0u) return 0.0f;
   const f3 rd = normalize(next);
   if (S.emit_root == CG_REF_NONE) return light_pdf_linear(S, rd, pos);
@@ -709,8 +773,7 @@ CG_HD float light_pdf(const DevScene& S, f3 next, f3 pos, uint32_t* stk, int str
       const bool hl = box_enter(pos, inv, nd->lo_l, nd->hi_l) < __builtin_inff();
       const bool hr = box_enter(pos, inv, nd->lo_r, nd->hi_r) < __builtin_inff();
       if (hl && hr) {
-        stk[sp * stride] = nd->ref_r;
-        ++sp;
+        stk.push(sp, nd->ref_r, 0.0f);
         ref = nd->ref_l;
         continue;
       }
@@ -737,8 +800,8 @@ CG_HD float light_pdf(const DevScene& S, f3 next, f3 pos, uint32_t* stk, int str
       }
     }
     if (sp == 0) break;
-    --sp;
-    ref = stk[sp * stride];
+    float unused;
+    stk.pop(sp, ref, unused);
   }
   float pdf = 0.0f;
   for (int k = 0; k < n; ++k) {
@@ -822,13 +885,13 @@ enum { kNone = 0, kDiffuse, kSpecR, kRoughR, kSpecT, kRoughT };
 // The light pdf multiplies a zero count everywhere below the first bounce;
 // it is finite for finite emitters, so it is only evaluated when counted.
 CG_HD float balance_den(const DevScene& S, f3 d, f3 next, const Surface& s, const Mat& m, const Counts6& c,
-                        int model, float model_pdf, float light, uint32_t* stk, int stride) {
+                        int model, float model_pdf, float light, const TStack& stk) {
   float pd = model == kDiffuse ? model_pdf : diffuse_pdf(d, next, s);
   float psr = model == kSpecR ? model_pdf : spec_reflect_pdf(d, next, s);
   float prr = model == kRoughR ? model_pdf : rough_reflect_pdf(d, next, s, m);
   float pst = model == kSpecT ? model_pdf : spec_trans_pdf(d, next, s, m);
   float prt = model == kRoughT ? model_pdf : rough_trans_pdf(d, next, s, m);
-  float pl = light >= 0.0f ? light : (c.l != 0u ? light_pdf(S, next, s.pos, stk, stride) : 0.0f);
+  float pl = light >= 0.0f ? light : (c.l != 0u ? light_pdf(S, next, s.pos, stk) : 0.0f);
   return fmx(static_cast<float>(c.d) * pd + static_cast<float>(c.sr) * psr + static_cast<float>(c.rr) * prr +
                  static_cast<float>(c.st) * pst + static_cast<float>(c.rt) * prt + static_cast<float>(c.l) * pl,
              1e-6f);
@@ -844,7 +907,7 @@ struct Frame5 {
   uint32_t tcount, ecount, iter;
 };
 CG_HD void forward_pass(const DevScene& S, f3 d, float l, const SmitsBin& b, const Surface& s, int depth,
-                        const TraceParams& P, uint32_t& rng, Frame5& x, uint32_t* stk, int stride) {
+                        const TraceParams& P, uint32_t& rng, Frame5& x, const TStack& stk) {
   const MatRec& pm = S.mats[umin(s.material, S.mat_count - 1u)];
   Mat m = decode(pm, l, b);
   Counts6 c = lobe_counts(m, S.alias_count != 0u, depth, P);
@@ -856,33 +919,33 @@ CG_HD void forward_pass(const DevScene& S, f3 d, float l, const SmitsBin& b, con
   for (uint32_t i = 0; i < c.d; ++i) {
     TSample t = diffuse_sample(d, s, rng);
     x.tdir[x.tcount] = t.dir;
-    x.tcoeff[x.tcount] = diffuse_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kDiffuse, t.prob, -1.0f, stk, stride);
+    x.tcoeff[x.tcount] = diffuse_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kDiffuse, t.prob, -1.0f, stk);
     x.tcount++;
   }
   for (uint32_t i = 0; i < c.sr; ++i) {
     f3 nd = specular_dir(d, s);
     float prob = spec_reflect_pdf(d, nd, s);
     x.tdir[x.tcount] = nd;
-    x.tcoeff[x.tcount] = spec_reflect_dist(d, nd, s, m) / balance_den(S, d, nd, s, m, c, kSpecR, prob, -1.0f, stk, stride);
+    x.tcoeff[x.tcount] = spec_reflect_dist(d, nd, s, m) / balance_den(S, d, nd, s, m, c, kSpecR, prob, -1.0f, stk);
     x.tcount++;
   }
   for (uint32_t i = 0; i < c.rr; ++i) {
     TSample t = rough_reflect_sample(d, s, m, rng);
     x.tdir[x.tcount] = t.dir;
-    x.tcoeff[x.tcount] = rough_reflect_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kRoughR, t.prob, -1.0f, stk, stride);
+    x.tcoeff[x.tcount] = rough_reflect_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kRoughR, t.prob, -1.0f, stk);
     x.tcount++;
   }
   for (uint32_t i = 0; i < c.st; ++i) {
     f3 nd = spec_trans_dir(d, s, m);
     float prob = spec_trans_pdf(d, nd, s, m);
     x.tdir[x.tcount] = nd;
-    x.tcoeff[x.tcount] = spec_trans_dist(d, nd, s, m) / balance_den(S, d, nd, s, m, c, kSpecT, prob, -1.0f, stk, stride);
+    x.tcoeff[x.tcount] = spec_trans_dist(d, nd, s, m) / balance_den(S, d, nd, s, m, c, kSpecT, prob, -1.0f, stk);
     x.tcount++;
   }
   for (uint32_t i = 0; i < c.rt; ++i) {
     TSample t = rough_trans_sample(d, s, m, rng);
     x.tdir[x.tcount] = t.dir;
-    x.tcoeff[x.tcount] = rough_trans_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kRoughT, t.prob, -1.0f, stk, stride);
+    x.tcoeff[x.tcount] = rough_trans_dist(d, t.dir, s, m) / balance_den(S, d, t.dir, s, m, c, kRoughT, t.prob, -1.0f, stk);
     x.tcount++;
   }
   for (uint32_t i = 0; i < c.l; ++i) {
@@ -891,7 +954,7 @@ CG_HD void forward_pass(const DevScene& S, f3 d, float l, const SmitsBin& b, con
     float proj = CG_FABS(dot(s.sn, e.dir));
     float coeff = (diffuse_dist(d, e.dir, s, m) + rough_reflect_dist(d, e.dir, s, m)) + rough_trans_dist(d, e.dir, s, m);
     x.edir = e.dir;
-    x.erad = e.radiance * coeff * proj / balance_den(S, d, e.dir, s, m, c, kNone, -1.0f, e.prob, stk, stride);
+    x.erad = e.radiance * coeff * proj / balance_den(S, d, e.dir, s, m, c, kNone, -1.0f, e.prob, stk);
     x.edist = e.dist;
     x.ecount++;
   }

@@ -32,7 +32,7 @@ struct DevBuf {
 }  // namespace
 
 struct cg_scene {
-  DevBuf nodes, tris, insts, vidx, vattr, mats, emit, alias, emit_order, env;
+  DevBuf bvh, nodes, tris, insts, vidx, vattr, mats, emit, alias, emit_order, env;
   DevScene view{};
   uint32_t stack_depth = 0;
   uint64_t bytes = 0;
@@ -44,7 +44,7 @@ struct cg_env {
   hipStream_t stream = nullptr;
   std::string last_error;
   DevBuf history, rgba, path_hash, ray_count, stats;
-  DevBuf wl_cdf, wl_leaf, queue;
+  DevBuf wl_cdf, wl_leaf, queue, ovf;
   int cu_count = 0;
   float wl_avg = 0.0f;
   bool counting = false;
@@ -93,28 +93,30 @@ int dev_upload(cg_env* env, DevBuf& b, const void* src, size_t bytes) {
 }
 
 // Work distribution.  Blocks are issued round-robin over the 8 XCDs
-// (block b runs on XCD b % 8).  Work items are split into 8 contiguous
-// ranges, one per XCD, so an XCD's L2 sees neighbouring rays or pixels; each
-// block takes items from its own XCD's range through an atomic counter and,
-// once that range is exhausted, steals from the next XCD's.  The grid is the
-// resident size, so blocks loop until every range is empty.
+// (block b runs on XCD b % 8).  Work items (64 rays, or an 8x8 pixel tile:
+// one wave's worth) are split into 8 contiguous ranges, one per XCD, so an
+// XCD's L2 sees neighbouring rays or pixels.  Each wave takes items from its
+// own XCD's range through an atomic counter and, once that range is
+// exhausted, steals from the next XCD's.  The grid is the resident size, so
+// waves loop until every range is empty; waves never wait for each other.
 constexpr uint32_t kXcds = 8u;
 constexpr uint32_t kNoItem = 0xFFFFFFFFu;
+constexpr uint32_t kCounterStride = 32u;  // one 128-byte line per counter
 struct WorkQueue {
-  uint32_t* counters;  // kXcds counters, zero at launch
+  uint32_t* counters;  // kXcds counters, kCounterStride apart, zero at launch
   uint32_t total;      // items
 };
-__device__ __forceinline__ uint32_t grab_item(const WorkQueue& q, uint32_t& probe, uint32_t* s_item) {
-  if (threadIdx.x == 0) {
+__device__ __forceinline__ uint32_t grab_item(const WorkQueue& q, uint32_t& probe) {
+  uint32_t item = kNoItem;
+  if ((threadIdx.x & 63u) == 0u) {
     const uint32_t per = (q.total + kXcds - 1u) / kXcds;
     const uint32_t home = blockIdx.x % kXcds;
-    uint32_t item = kNoItem;
     while (probe < kXcds) {
       const uint32_t x = (home + probe) % kXcds;
       const uint32_t begin = x * per;
       const uint32_t end = begin + per < q.total ? begin + per : q.total;
       if (begin < end) {
-        const uint32_t c = atomicAdd(q.counters + x, 1u);
+        const uint32_t c = atomicAdd(q.counters + x * kCounterStride, 1u);
         if (begin + c < end) {
           item = begin + c;
           break;
@@ -122,12 +124,8 @@ __device__ __forceinline__ uint32_t grab_item(const WorkQueue& q, uint32_t& prob
       }
       ++probe;
     }
-    *s_item = item;
   }
-  __syncthreads();
-  const uint32_t item = *s_item;
-  __syncthreads();
-  return item;
+  return __shfl(item, 0);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
@@ -156,35 +154,47 @@ __device__ __forceinline__ void flush_stats(unsigned long long* stats, uint64_t 
 }
 
 // ---------------------------------------------------------------------------
-// Batch traversal: one ray per lane, traversal stack in LDS interleaved by
-// lane (entry k of lane i at k * 256 + i: conflict-free).
+// Batch traversal: one ray per lane, the short traversal stack in LDS
+// interleaved by lane (entry k of lane i at k * 256 + i: conflict-free), the
+// overflow in device memory interleaved over the whole grid.
 constexpr int kBlock = 256;
 
-template <int SS, bool kAnyHit>
+struct StackArgs {
+  uint2* ovf;
+  int ovf_cap;
+};
+__device__ __forceinline__ TStack lane_stack(uint32_t* s_ref, float* s_t, const StackArgs& sa) {
+  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+  return TStack{s_ref + threadIdx.x, s_t + threadIdx.x, kBlock, sa.ovf ? sa.ovf + g : nullptr, gridDim.x * kBlock,
+                sa.ovf ? sa.ovf_cap : 0};
+}
+
+template <bool kAnyHit>
 __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const cg_ray* __restrict__ rays,
                                                   const float* __restrict__ tmax, uint32_t n,
                                                   cg_hit* __restrict__ hits, float* __restrict__ uv,
                                                   uint32_t* __restrict__ occluded,
-                                                  unsigned long long* __restrict__ stats, WorkQueue q) {
-  __shared__ uint32_t stk[SS * kBlock];
-  __shared__ uint32_t s_item;
+                                                  unsigned long long* __restrict__ stats, WorkQueue q, StackArgs sa) {
+  __shared__ uint32_t s_ref[kShortStack * kBlock];
+  __shared__ float s_t[kShortStack * kBlock];
+  const TStack stk = lane_stack(s_ref, s_t, sa);
   TravStats st{0u, 0u, 0u};
   uint64_t nq = 0;
   uint32_t probe = 0;
   for (;;) {
-    const uint32_t item = grab_item(q, probe, &s_item);
+    const uint32_t item = grab_item(q, probe);
     if (item == kNoItem) break;
-    const uint32_t i = item * kBlock + threadIdx.x;
+    const uint32_t i = item * 64u + (threadIdx.x & 63u);
     if (i >= n) continue;
     ++nq;
     const float4* rp = reinterpret_cast<const float4*>(rays + i);
     float4 a = rp[0], b = rp[1];
     f3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z);
     if (kAnyHit) {
-      HitRec h = traverse<true>(S, o, d, fmn(tmax[i], kMax), stk + threadIdx.x, kBlock, st);
+      HitRec h = traverse<true>(S, o, d, fmn(tmax[i], kMax), stk, st);
       occluded[i] = h.inst != CG_MISS_ID ? 1u : 0u;
     } else {
-      HitRec h = traverse<false>(S, o, d, kMax, stk + threadIdx.x, kBlock, st);
+      HitRec h = traverse<false>(S, o, d, kMax, stk, st);
       cg_hit out;
       if (h.inst != CG_MISS_ID) {
         out.t = h.t;
@@ -210,21 +220,24 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const cg_ray* __re
 // The compute pass (compute.wesl:32-62, render.wesl:7-18, trace.wesl:22-113)
 // with one lane per pixel, so each pixel consumes its RNG stream in exactly
 // the reference's depth-first order.  The transport tree's open frames live
-// in a private array (MF frames = trace_depth - 1); the BVH stack in LDS.
-template <int SS, int MF>
-__global__ __launch_bounds__(kBlock, 4) void k_view(ViewArgs A, WorkQueue q) {
-  __shared__ uint32_t stk_lds[SS * kBlock];
-  __shared__ uint32_t s_item;
-  const uint32_t tiles_x = (A.W + 15u) / 16u;
+// in a private array (MF frames = trace_depth - 1); the BVH stack in LDS
+// (overflow in device memory).
+template <int MF>
+__global__ __launch_bounds__(kBlock, 4) void k_view(ViewArgs A, WorkQueue q, StackArgs sa) {
+  __shared__ uint32_t s_ref[kShortStack * kBlock];
+  __shared__ float s_t[kShortStack * kBlock];
+  const TStack stk = lane_stack(s_ref, s_t, sa);
+  const uint32_t tiles_x = (A.W + 7u) / 8u;
   TravStats st{0u, 0u, 0u};
   uint64_t nrays = 0, nshadow = 0;
   uint32_t probe = 0;
+  const uint32_t lane = threadIdx.x & 63u;
   for (;;) {
-    const uint32_t tile = grab_item(q, probe, &s_item);
+    const uint32_t tile = grab_item(q, probe);
     if (tile == kNoItem) break;
-    const uint32_t px = (tile % tiles_x) * 16u + (threadIdx.x & 15u);
-    const uint32_t py = (tile / tiles_x) * 16u + (threadIdx.x >> 4);
-    if (px < A.W && py < A.H) render_pixel<MF>(A, px, py, stk_lds + threadIdx.x, kBlock, st, nrays, nshadow);
+    const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u);
+    const uint32_t py = (tile / tiles_x) * 8u + (lane >> 3);
+    if (px < A.W && py < A.H) render_pixel<MF>(A, px, py, stk, st, nrays, nshadow);
   }
   if (A.stats) flush_stats(A.stats, nrays, nshadow, st.nodes, st.tris, st.insts);
 }
@@ -243,6 +256,7 @@ int upload_vec(cg_env* env, DevBuf& b, const std::vector<T>& v, uint64_t& total)
 }
 
 void release_scene(cg_scene* s) {
+  dev_free(s->bvh);
   dev_free(s->nodes);
   dev_free(s->tris);
   dev_free(s->insts);
@@ -266,33 +280,57 @@ uint32_t resident_grid(K kernel, const cg_env* env, uint32_t items) {
 }
 WorkQueue reset_queue(cg_env* env, uint32_t items) {
   uint32_t* c = static_cast<uint32_t*>(env->queue.p);
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kXcds), 0, env->stream, c, kXcds);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kXcds * kCounterStride), 0, env->stream, c, kXcds * kCounterStride);
   return WorkQueue{c, items};
 }
-template <int SS, int MF>
-void launch_view_mf(cg_env* env, const ViewArgs& a, uint32_t tiles) {
+// Overflow stack area for a launch of `grid` blocks on a scene whose
+// traversal stack is at most `depth` entries deep.
+int stack_args(cg_env* env, uint32_t grid, uint32_t depth, StackArgs& sa) {
+  sa.ovf = nullptr;
+  sa.ovf_cap = 0;
+  if (depth <= static_cast<uint32_t>(kShortStack)) return CG_OK;
+  const int cap = static_cast<int>(depth) - kShortStack;
+  const size_t bytes = static_cast<size_t>(grid) * kBlock * static_cast<size_t>(cap) * sizeof(uint2);
+  if (env->ovf.bytes < bytes) {
+    if (env->ovf.p) {
+      CG_HIP(env, hipStreamSynchronize(env->stream));
+      dev_free(env->ovf);
+    }
+    int rc = dev_alloc(env, env->ovf, bytes);
+    if (rc != CG_OK) return rc;
+  }
+  sa.ovf = static_cast<uint2*>(env->ovf.p);
+  sa.ovf_cap = cap;
+  return CG_OK;
+}
+template <int MF>
+int launch_view_mf(cg_env* env, const ViewArgs& a, uint32_t tiles, uint32_t depth) {
+  const uint32_t grid = resident_grid(k_view<MF>, env, (tiles + 3u) / 4u);
+  StackArgs sa;
+  int rc = stack_args(env, grid, depth, sa);
+  if (rc != CG_OK) return rc;
   WorkQueue q = reset_queue(env, tiles);
-  hipLaunchKernelGGL((k_view<SS, MF>), dim3(resident_grid(k_view<SS, MF>, env, tiles)), dim3(kBlock), 0, env->stream, a,
-                     q);
+  hipLaunchKernelGGL((k_view<MF>), dim3(grid), dim3(kBlock), 0, env->stream, a, q, sa);
+  return CG_OK;
 }
-template <int SS>
-void launch_view_ss(cg_env* env, const ViewArgs& a, int frames, uint32_t tiles) {
-  if (frames <= 4)
-    launch_view_mf<SS, 4>(env, a, tiles);
-  else if (frames <= 8)
-    launch_view_mf<SS, 8>(env, a, tiles);
-  else if (frames <= 16)
-    launch_view_mf<SS, 16>(env, a, tiles);
-  else
-    launch_view_mf<SS, 64>(env, a, tiles);
+int launch_view(cg_env* env, const ViewArgs& a, int frames, uint32_t tiles, uint32_t depth) {
+  if (frames <= 4) return launch_view_mf<4>(env, a, tiles, depth);
+  if (frames <= 8) return launch_view_mf<8>(env, a, tiles, depth);
+  if (frames <= 16) return launch_view_mf<16>(env, a, tiles, depth);
+  return launch_view_mf<64>(env, a, tiles, depth);
 }
-template <int SS, bool kAnyHit>
-void launch_trace(cg_env* env, const DevScene& S, const cg_ray* rays, const float* tmax, uint32_t n, cg_hit* hits,
-                  float* uv, uint32_t* occ, unsigned long long* st) {
-  const uint32_t items = (n + kBlock - 1) / kBlock;
+template <bool kAnyHit>
+int launch_trace(cg_env* env, const cg_scene* scene, const cg_ray* rays, const float* tmax, uint32_t n, cg_hit* hits,
+                 float* uv, uint32_t* occ, unsigned long long* st) {
+  const uint32_t items = (n + 63u) / 64u;
+  const uint32_t grid = resident_grid(k_trace<kAnyHit>, env, (items + 3u) / 4u);
+  StackArgs sa;
+  int rc = stack_args(env, grid, scene->stack_depth, sa);
+  if (rc != CG_OK) return rc;
   WorkQueue q = reset_queue(env, items);
-  hipLaunchKernelGGL((k_trace<SS, kAnyHit>), dim3(resident_grid(k_trace<SS, kAnyHit>, env, items)), dim3(kBlock), 0,
-                     env->stream, S, rays, tmax, n, hits, uv, occ, st, q);
+  hipLaunchKernelGGL((k_trace<kAnyHit>), dim3(grid), dim3(kBlock), 0, env->stream, scene->view, rays, tmax, n, hits, uv,
+                     occ, st, q, sa);
+  return CG_OK;
 }
 
 }  // namespace
@@ -331,7 +369,7 @@ int cg_env_create(const cg_conf* conf, int device, cg_env** out) {
   if ((rc = dev_alloc(env, env->history, px * sizeof(cg_history_pixel))) != CG_OK) return bail(rc);
   if ((rc = dev_alloc(env, env->rgba, px * 4)) != CG_OK) return bail(rc);
   if ((rc = dev_alloc(env, env->stats, 8 * sizeof(unsigned long long))) != CG_OK) return bail(rc);
-  if ((rc = dev_alloc(env, env->queue, kXcds * sizeof(uint32_t))) != CG_OK) return bail(rc);
+  if ((rc = dev_alloc(env, env->queue, kXcds * kCounterStride * sizeof(uint32_t))) != CG_OK) return bail(rc);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return bail(fail(env, CG_ERR_HIP, "hipGetDeviceProperties failed"));
   env->cu_count = prop.multiProcessorCount;
@@ -365,6 +403,7 @@ int cg_env_destroy(cg_env* env) {
   dev_free(env->wl_cdf);
   dev_free(env->wl_leaf);
   dev_free(env->queue);
+  dev_free(env->ovf);
   if (env->stream) (void)hipStreamDestroy(env->stream);
   delete env;
   return CG_OK;
@@ -430,13 +469,14 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
   std::string err;
   int rc = build_layout(sd, L, err, env->accel);
   if (rc != CG_OK) return fail(env, rc, "%s", err.c_str());
-  if (L.stack_depth > 64u)
-    return fail(env, CG_ERR_UNSUPPORTED, "BVH needs a %u-entry traversal stack (max 64)", L.stack_depth);
+  if (L.stack_depth > 4096u)
+    return fail(env, CG_ERR_UNSUPPORTED, "BVH needs a %u-entry traversal stack (max 4096)", L.stack_depth);
   cg_scene* s = new cg_scene();
   uint64_t total = 0;
   auto up = [&](int r) {
     if (r != CG_OK && rc == CG_OK) rc = r;
   };
+  up(upload_vec(env, s->bvh, L.bvh, total));
   up(upload_vec(env, s->nodes, L.nodes, total));
   up(upload_vec(env, s->tris, L.tris, total));
   up(upload_vec(env, s->insts, L.insts, total));
@@ -457,6 +497,7 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
     return rc;
   }
   DevScene& v = s->view;
+  v.bvh = static_cast<const Node4*>(s->bvh.p);
   v.nodes = static_cast<const PairNode*>(s->nodes.p);
   v.tris = static_cast<const TriRec*>(s->tris.p);
   v.insts = static_cast<const InstRec*>(s->insts.p);
@@ -526,12 +567,10 @@ int cg_view(cg_env* env, cg_scene* scene, const cg_camera* camera, uint32_t iter
   a.path_hash = env->path_records ? static_cast<uint32_t*>(env->path_hash.p) : nullptr;
   a.ray_count = env->path_records ? static_cast<uint32_t*>(env->ray_count.p) : nullptr;
   a.stats = env->counting ? static_cast<unsigned long long*>(env->stats.p) : nullptr;
-  const uint32_t tiles = ((a.W + 15u) / 16u) * ((a.H + 15u) / 16u);
+  const uint32_t tiles = ((a.W + 7u) / 8u) * ((a.H + 7u) / 8u);
   const int frames = std::max(1, a.P.D - 1);
-  if (scene->stack_depth <= 32u)
-    launch_view_ss<32>(env, a, frames, tiles);
-  else
-    launch_view_ss<64>(env, a, frames, tiles);
+  int rc = launch_view(env, a, frames, tiles, scene->stack_depth);
+  if (rc != CG_OK) return rc;
   CG_HIP(env, hipGetLastError());
   return CG_OK;
 }
@@ -566,10 +605,8 @@ int cg_trace_closest(cg_env* env, cg_scene* scene, const cg_ray* rays, uint32_t 
   if (!env || !scene || (!rays && n) || (!hits && n)) return fail(env, CG_ERR_INVALID, "cg_trace_closest: null argument");
   if (n == 0) return CG_OK;
   unsigned long long* st = env->counting ? static_cast<unsigned long long*>(env->stats.p) : nullptr;
-  if (scene->stack_depth <= 32u)
-    launch_trace<32, false>(env, scene->view, rays, nullptr, n, hits, uv, nullptr, st);
-  else
-    launch_trace<64, false>(env, scene->view, rays, nullptr, n, hits, uv, nullptr, st);
+  int rc = launch_trace<false>(env, scene, rays, nullptr, n, hits, uv, nullptr, st);
+  if (rc != CG_OK) return rc;
   CG_HIP(env, hipGetLastError());
   return CG_OK;
 }
@@ -580,10 +617,8 @@ int cg_trace_occluded(cg_env* env, cg_scene* scene, const cg_ray* rays, const fl
     return fail(env, CG_ERR_INVALID, "cg_trace_occluded: null argument");
   if (n == 0) return CG_OK;
   unsigned long long* st = env->counting ? static_cast<unsigned long long*>(env->stats.p) : nullptr;
-  if (scene->stack_depth <= 32u)
-    launch_trace<32, true>(env, scene->view, rays, tmax, n, nullptr, nullptr, occluded, st);
-  else
-    launch_trace<64, true>(env, scene->view, rays, tmax, n, nullptr, nullptr, occluded, st);
+  int rc = launch_trace<true>(env, scene, rays, tmax, n, nullptr, nullptr, occluded, st);
+  if (rc != CG_OK) return rc;
   CG_HIP(env, hipGetLastError());
   return CG_OK;
 }

@@ -4,7 +4,10 @@
 // Every record is a whole number of 16-byte vectors and either 64 B or a
 // multiple of 16 B, so one traversal step is a handful of dwordx4 loads from
 // one or two 128-byte lines:
-//   PairNode  64 B  both child boxes of one interior BVH node + child refs
+//   Node4    128 B  one 4-wide BVH node (one 128-byte line): the boxes of up
+//                   to four children in SoA order, their refs
+//   PairNode  64 B  both child boxes of one interior node of the binary
+//                   emitter BVH + child refs
 //   TriRec    48 B  v0, e1 = v1 - v0, e2 = v2 - v0, source triangle id (and,
 //                   in the merged world BLAS, the triangle's instance)
 //   InstRec   80 B  full column-major inverse transform, BLAS root ref, material
@@ -17,7 +20,8 @@
 #include <stdint.h>
 
 // Child / root references (32 bits):
-//   00 | index       interior pair node (30-bit index)
+//   00 | index       interior node (30-bit index: Node4 in the scene BVH,
+//                    PairNode in the emitter BVH)
 //   01 | index       TLAS leaf: instance index (30-bit)
 //   1 cc | slot      BLAS leaf: cc + 1 triangles (1-4) from slot (29-bit)
 //   kRefNone         empty (BLAS with no triangles, or empty TLAS)
@@ -31,6 +35,13 @@
 #define CG_REF_MARKER 0x7FFFFFFFu
 #define CG_LEAF_MAX 4u
 #define CG_MAKE_LEAF(slot, count) (CG_REF_LEAF | (((count) - 1u) << CG_REF_COUNT_SHIFT) | (slot))
+
+struct __attribute__((aligned(16))) Node4 {
+  float lox[4], loy[4], loz[4];  // child boxes; an empty slot has lo = +inf,
+  float hix[4], hiy[4], hiz[4];  // hi = -inf and ref CG_REF_NONE
+  uint32_t ref[4];
+  uint32_t _pad[4];
+};
 
 struct __attribute__((aligned(16))) PairNode {
   float lo_l[3];   // left child box
@@ -101,6 +112,7 @@ struct __attribute__((aligned(16))) WlLeaf {
   float _pad[3];
 };
 
+static_assert(sizeof(Node4) == 128, "Node4");
 static_assert(sizeof(PairNode) == 64, "PairNode");
 static_assert(sizeof(TriRec) == 48, "TriRec");
 static_assert(sizeof(InstRec) == 80, "InstRec");

@@ -29,7 +29,7 @@ struct ViewArgs {
 
 
 template <int MF>
-CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, uint32_t* stk, int stride, TravStats& st,
+CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, const TStack& stk, TravStats& st,
                         uint64_t& nrays, uint64_t& nshadow) {
   const DevScene& S = A.S;
   f3 co, cd;
@@ -61,7 +61,7 @@ CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, uint32_t* s
     f3 ro = co, rd = cd;
     float ret = 0.0f;
     for (;;) {
-      HitRec h = traverse<false>(S, ro, rd, kMax, stk, stride, st);
+      HitRec h = traverse<false>(S, ro, rd, kMax, stk, st);
       ++nrays;
       hash = hmix(hmix(hash, h.inst), h.inst != CG_MISS_ID ? S.tris[h.slot].tri_id : CG_MISS_ID);
       bool descended = false;
@@ -74,12 +74,12 @@ CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, uint32_t* s
       } else {
         Surface sf = surface_at(S, ro, rd, h);
         Frame5& f = F[depth];
-        forward_pass(S, rd, L.lambda, sb, sf, depth, A.P, rng, f, stk, stride);
+        forward_pass(S, rd, L.lambda, sb, sf, depth, A.P, rng, f, stk);
         if (f.ecount != 0u) {  // solveEmission (trace.wesl:91-113)
           f3 lp = sf.pos + f.edir * f.edist;
           f3 so = spawn_origin(sf.pos, sf.gn, f.edir);
           float expected = length(so - lp);
-          HitRec sh = traverse<true>(S, so, f.edir, fmn(expected - 1e-3f, kMax), stk, stride, st);
+          HitRec sh = traverse<true>(S, so, f.edir, fmn(expected - 1e-3f, kMax), stk, st);
           ++nshadow;
           bool occ = sh.inst != CG_MISS_ID;
           hash = hmix(hash, 0xA5000000u | (occ ? 1u : 0u));

@@ -132,7 +132,8 @@ inline void box_of(const float* lb, const float* ub, float* lo, float* hi) {
 }
 
 struct Layout {
-  std::vector<PairNode> nodes;
+  std::vector<Node4> bvh;       // scene BVH (TLAS and BLASes), 4-wide
+  std::vector<PairNode> nodes;  // binary trees while building; then the emitter BVH
   std::vector<TriRec> tris;
   std::vector<InstRec> insts;
   std::vector<uint4> vidx;
@@ -807,12 +808,120 @@ inline void build_emitter_bvh(Layout& L) {
   L.stack_depth = std::max(L.stack_depth, t.depth + 1u);
 }
 
+// ---------------------------------------------------------------------------
+// 4-wide BVH from a binary one: each 4-wide node takes an interior binary
+// node's two children and repeatedly replaces the interior child of largest
+// surface area by its own two children, up to four.  Every child box is a
+// box of the binary tree, bit for bit, so each box test decides as it did in
+// the binary traversal; a traversal visits about half as many nodes, each one
+// 128-byte line.  Nodes are stored parent first, depth first.
+struct Collapse4 {
+  const std::vector<PairNode>& pairs;
+  std::vector<Node4>& out;
+  std::vector<uint32_t> map;
+  explicit Collapse4(const std::vector<PairNode>& p, std::vector<Node4>& o)
+      : pairs(p), out(o), map(p.size(), CG_REF_NONE) {}
+  static bool interior(uint32_t r) { return r != CG_REF_NONE && (r & (CG_REF_LEAF | CG_REF_INST)) == 0u; }
+  static float half_area(const float* lo, const float* hi) {
+    const float x = hi[0] - lo[0], y = hi[1] - lo[1], z = hi[2] - lo[2];
+    return x * y + y * z + z * x;
+  }
+  struct Child {
+    uint32_t ref;
+    float lo[3], hi[3];
+  };
+  static Child left(const PairNode& p) {
+    return Child{p.ref_l, {p.lo_l[0], p.lo_l[1], p.lo_l[2]}, {p.hi_l[0], p.hi_l[1], p.hi_l[2]}};
+  }
+  static Child right(const PairNode& p) {
+    return Child{p.ref_r, {p.lo_r[0], p.lo_r[1], p.lo_r[2]}, {p.hi_r[0], p.hi_r[1], p.hi_r[2]}};
+  }
+  uint32_t convert(uint32_t r) {
+    if (!interior(r)) return r;
+    if (map[r] != CG_REF_NONE) return map[r];
+    Child ch[4];
+    int n = 2;
+    ch[0] = left(pairs[r]);
+    ch[1] = right(pairs[r]);
+    while (n < 4) {
+      int best = -1;
+      float ba = -1.0f;
+      for (int i = 0; i < n; ++i)
+        if (interior(ch[i].ref)) {
+          const float a = half_area(ch[i].lo, ch[i].hi);
+          if (a > ba) {
+            ba = a;
+            best = i;
+          }
+        }
+      if (best < 0) break;
+      const PairNode& q = pairs[ch[best].ref];
+      ch[best] = left(q);
+      ch[n++] = right(q);
+    }
+    const uint32_t idx = static_cast<uint32_t>(out.size());
+    map[r] = idx;
+    out.emplace_back();
+    Node4 nd;
+    for (int i = 0; i < 4; ++i) {
+      const bool used = i < n;
+      const float inf = __builtin_inff();
+      nd.lox[i] = used ? ch[i].lo[0] : inf;
+      nd.loy[i] = used ? ch[i].lo[1] : inf;
+      nd.loz[i] = used ? ch[i].lo[2] : inf;
+      nd.hix[i] = used ? ch[i].hi[0] : -inf;
+      nd.hiy[i] = used ? ch[i].hi[1] : -inf;
+      nd.hiz[i] = used ? ch[i].hi[2] : -inf;
+      nd.ref[i] = CG_REF_NONE;
+      nd._pad[i] = 0u;
+    }
+    for (int i = 0; i < n; ++i) nd.ref[i] = convert(ch[i].ref);
+    out[idx] = nd;
+    return idx;
+  }
+};
+
+// Deepest traversal stack of a 4-wide tree: a node with k children pushes at
+// most k - 1 of them and descends into the nearest; entering an instance
+// pushes one marker.
+inline uint32_t stack_bound4(const std::vector<Node4>& bvh, const std::vector<InstRec>& insts, uint32_t r,
+                             std::vector<uint32_t>& memo) {
+  if (r == CG_REF_NONE || (r & CG_REF_LEAF) != 0u) return 0u;
+  if ((r & CG_REF_INST) != 0u) {
+    const InstRec& in = insts[r & CG_REF_INDEX_MASK];
+    return in.root == CG_REF_NONE ? 0u : 1u + stack_bound4(bvh, insts, in.root, memo);
+  }
+  if (memo[r] != ~0u) return memo[r];
+  const Node4& n = bvh[r];
+  uint32_t k = 0, deepest = 0;
+  for (int i = 0; i < 4; ++i)
+    if (n.ref[i] != CG_REF_NONE) {
+      ++k;
+      deepest = std::max(deepest, stack_bound4(bvh, insts, n.ref[i], memo));
+    }
+  return memo[r] = (k > 0 ? k - 1u : 0u) + deepest;
+}
+
+inline void collapse_scene_bvh(Layout& L) {
+  L.bvh.clear();
+  L.bvh.reserve(L.nodes.size() / 2 + 1);
+  Collapse4 c(L.nodes, L.bvh);
+  for (InstRec& in : L.insts) in.root = c.convert(in.root);
+  L.root = c.convert(L.root);
+  std::vector<uint32_t> memo(L.bvh.size(), ~0u);
+  L.stack_depth = stack_bound4(L.bvh, L.insts, L.root, memo);
+  L.nodes.clear();
+  L.nodes.shrink_to_fit();
+}
+
 // accel 0: the reference's trees; accel 1: rebuilt (SAH, merged world BLAS).
+// Either is traversed as a 4-wide BVH.
 inline int build_layout(const cg_scene_data* sd, Layout& L, std::string& err, int accel = 1) {
   int rc = build_common(sd, L, err);
   if (rc != CG_OK) return rc;
   rc = accel == 0 ? build_reference_accel(sd, L, err) : build_sah_accel(sd, L, err);
   if (rc != CG_OK) return rc;
+  collapse_scene_bvh(L);
   build_emitter_bvh(L);
   return CG_OK;
 }
