@@ -225,9 +225,17 @@ def main() -> int:
     micro = None
     if not args.no_traversal:
         micro = traversal_microbench(cg, ev, 1 << 24, 10_000_000, args.traversal_reps)
-        if probe:
-            for k in ("all", "coherent", "incoherent"):
-                micro[k]["roofline"]["frac_of_measured"] = round(micro[k]["roofline"]["achieved"] / probe["GB/s"], 4)
+        tprof = os.path.join(ROOT, "profiles", "k_trace_c2_2p24rays_10Mtris.json")
+        tdata = None
+        if os.path.exists(tprof):
+            with open(tprof) as f:
+                tdata = json.load(f)
+        for k in ("all", "coherent", "incoherent"):
+            rl = micro[k]["roofline"]
+            if probe:
+                rl["frac_of_measured"] = round(rl["achieved"] / probe["GB/s"], 4)
+            rl["traffic"] = round(tdata[k]["traffic_bytes"]) if tdata else None
+            rl["traffic_source"] = os.path.relpath(tprof, ROOT) if tdata else None
 
     base = None
     if not args.no_cpu:
