@@ -35,6 +35,7 @@ CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, const TStac
   f3 co, cd;
   camera_ray(A.cam_pos, A.cam_dir, A.viewport, A.W, A.H, px, py, co, cd);
   const uint32_t hidx = px + py * A.W;
+  const uint64_t rays_before = nrays + nshadow;  // the lane's counters span several pixels
   const float w = 1.0f / static_cast<float>(A.iter + 1u);
   f3 lum = mk3(0.0f, 0.0f, 0.0f);
   uint32_t rng = hidx;
@@ -133,7 +134,7 @@ CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, const TStac
   A.rgba[hidx] = xyz_to_rgba8(lum);
   if (A.path_hash) {
     A.path_hash[hidx] = hash;
-    A.ray_count[hidx] = static_cast<uint32_t>(nrays + nshadow);
+    A.ray_count[hidx] = static_cast<uint32_t>(nrays + nshadow - rays_before);
   }
 }
 

@@ -82,6 +82,8 @@ def _render_parity(cg, oracle, name, params, conf, frames=1, seed=13, accel=1):
     print(f"{name} {conf.render_width}x{conf.render_height} spp {conf.sample_count} D {conf.trace_depth} frames {frames}: "
           f"paths identical {same.mean()*100:.2f}%, max rel err (non-diverged) {err[same].max():.3e}, "
           f"ray count equal {np.mean(rcount == ref['ray_count'])*100:.2f}%")
+    # a pixel's path record and its ray count are fixed by the same walk
+    assert np.array_equal(rcount[same], ref["ray_count"][same])
     return same, err, hist, ref, rgba
 
 
@@ -100,6 +102,33 @@ def test_render_per_pixel(cg, oracle, name, params, w, h, spp, depth):
     assert np.array_equal(hist["rand_seed"][same], ref["history"]["rand_seed"][same])
     d = np.abs(rgba.astype(int) - ref["rgba"].astype(int))[same]
     assert d.max() <= 1
+
+
+def test_render_full_frame_bands(cg, oracle):
+    """A full 1280x720 frame: each wave renders several 8x8 tiles in turn,
+    so per-pixel state must not leak from one tile to the next.  Checked on
+    three bands of rows (top, middle, bottom) against the oracle."""
+    s = cg.HostScene.generate("c1", seed=5, params=[0.25])
+    sd = s.scene_data()
+    cam = s.describe()["camera"]
+    conf = cg.Conf.make(1280, 720, 1, 4)
+    orc = oracle.OracleScene.wrap(sd.c)
+    with cg.DeviceEnv(conf) as env:
+        ds = env.upload(sd)
+        env.set_path_records(True)
+        env.view(ds, cam, 0)
+        hist = env.read_history()
+        phash, rcount = env.read_path_records()
+    for y0 in (0, 352, 712):
+        ref = orc.render(conf, cam, rows=(y0, y0 + 8), threads=16)
+        sl = slice(y0, y0 + 8)
+        same = phash[sl] == ref["path_hash"][sl]
+        err = pixel_rel_err(hist["xyz"][sl], ref["history"]["xyz"][sl])
+        print(f"rows {y0}-{y0 + 8}: paths identical {same.mean()*100:.2f}%, max rel err {err[same].max():.3e}")
+        assert same.mean() >= 0.99
+        assert err[same].max() <= 1e-4
+        assert np.array_equal(rcount[sl][same], ref["ray_count"][sl][same])
+        assert np.array_equal(hist["rand_seed"][sl][same], ref["history"]["rand_seed"][sl][same])
 
 
 def test_render_reference_layout(cg, oracle):
