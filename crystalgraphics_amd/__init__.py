@@ -20,7 +20,7 @@ from . import native
 from .native import (ALIAS, BLAS_NODE, EMISSIVE, HISTORY, HIT, INDEX, INSTANCE, MATERIAL, MISS_ID, MISS_T,
                      PRIMITIVE, RAY, TLAS_NODE, VERTEX, Camera, Conf, NativeError)
 
-__all__ = ["HostScene", "SceneData", "DeviceEnv", "DeviceScene", "Camera", "Conf", "NativeError",
+__all__ = ["HostScene", "SceneData", "DeviceEnv", "DeviceScene", "Camera", "Conf", "NativeError", "Batch", "RenderJob",
            "EnvInit", "EnvTerm", "EnvPoll", "EnvPresent", "EnvStatus", "LoadScene", "View", "ViewToBuffer",
            "MISS_ID", "MISS_T", "RAY", "HIT", "HISTORY", "VERTEX", "MATERIAL", "PRIMITIVE", "TLAS_NODE",
            "INSTANCE", "BLAS_NODE", "INDEX", "EMISSIVE", "ALIAS"]
@@ -321,6 +321,112 @@ class DeviceEnv:
             self.close()
         except Exception:
             pass
+
+
+class RenderJob:
+    """One render job of a Batch: a progressive frame of one scene and camera
+    at its own conf, with device buffers for its history (in/out), rgba8
+    target and path records."""
+
+    def __init__(self, env: DeviceEnv, scene: DeviceScene, camera: Camera, conf: Conf, iter_count: int = 0,
+                 path_records: bool = False, history: np.ndarray | None = None):
+        self.env, self.scene, self.camera, self.conf, self.iter_count = env, scene, camera, conf, iter_count
+        n = conf.render_width * conf.render_height
+        self.npix = n
+        self.history = env.alloc(n * HISTORY.itemsize)
+        if history is not None:
+            self.history.upload(np.ascontiguousarray(history, HISTORY).reshape(-1))
+        self.rgba = env.alloc(n * 4)
+        self.path_hash = env.alloc(n * 4) if path_records else None
+        self.ray_count = env.alloc(n * 4) if path_records else None
+        self.id = None
+
+    def c(self) -> native.Job:
+        j = native.Job()
+        j.scene = self.scene.handle
+        j.camera = self.camera
+        j.conf = self.conf
+        j.iter_count = self.iter_count
+        j.history = self.history.ptr
+        j.rgba8 = self.rgba.ptr
+        j.path_hash = self.path_hash.ptr if self.path_hash else None
+        j.ray_count = self.ray_count.ptr if self.ray_count else None
+        return j
+
+    def read_history(self) -> np.ndarray:
+        return self.history.download(HISTORY, self.npix).reshape(self.conf.render_height, self.conf.render_width)
+
+    def read_rgba8(self) -> np.ndarray:
+        return self.rgba.download(np.uint8, self.npix * 4).reshape(self.conf.render_height, self.conf.render_width, 4)
+
+    def read_path_records(self) -> tuple[np.ndarray, np.ndarray]:
+        h, w = self.conf.render_height, self.conf.render_width
+        return (self.path_hash.download(np.uint32, self.npix).reshape(h, w),
+                self.ray_count.download(np.uint32, self.npix).reshape(h, w))
+
+    def free(self) -> None:
+        for b in (self.history, self.rgba, self.path_hash, self.ray_count):
+            if b:
+                b.free()
+
+
+class Batch:
+    """The wavefront engine for many render jobs (cg_batch): jobs are admitted
+    between launches of `rounds_per_launch` rounds; each finishes with the
+    same per-pixel result as DeviceEnv.view."""
+
+    def __init__(self, env: DeviceEnv, slot_capacity: int = 1 << 20, max_trace_depth: int = 8,
+                 rounds_per_launch: int = 16, use_graph: bool = True, max_jobs: int = 256,
+                 max_stack_depth: int = 64):
+        self.env = env
+        self.conf = native.BatchConf(slot_capacity, max_trace_depth, rounds_per_launch, int(use_graph), max_jobs,
+                                     max_stack_depth)
+        h = C.c_void_p()
+        env._check(native.device().cg_batch_create(env.handle, C.byref(self.conf), C.byref(h)))
+        self._h = h.value
+
+    def submit(self, job: RenderJob) -> bool:
+        """Admit a job; False when there is no room now (CG_ERR_FULL)."""
+        jid = C.c_uint32()
+        rc = native.device().cg_batch_submit(self._h, C.byref(job.c()), C.byref(jid))
+        if rc == 6:
+            return False
+        self.env._check(rc)
+        job.id = jid.value
+        return True
+
+    def launch(self, launches: int = 1) -> None:
+        self.env._check(native.device().cg_batch_launch(self._h, launches))
+
+    def poll(self, cap: int = 4096) -> tuple[list[int], int]:
+        done = (C.c_uint32 * cap)()
+        nd, na = C.c_uint32(), C.c_uint32()
+        self.env._check(native.device().cg_batch_poll(self._h, done, cap, C.byref(nd), C.byref(na)))
+        return [done[i] for i in range(nd.value)], na.value
+
+    def drain(self) -> None:
+        self.env._check(native.device().cg_batch_drain(self._h))
+
+    def stats(self, reset: bool = False) -> dict:
+        s = native.Stats()
+        self.env._check(native.device().cg_batch_stats(self._h, C.byref(s), int(reset)))
+        return {k: int(getattr(s, k)) for k, _ in native.Stats._fields_}
+
+    def footprint(self) -> tuple[int, int]:
+        b, f = C.c_uint64(), C.c_uint32()
+        self.env._check(native.device().cg_batch_footprint(self._h, C.byref(b), C.byref(f)))
+        return b.value, f.value
+
+    def close(self) -> None:
+        if self._h:
+            native.device().cg_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 # --- the crystal::graphics API (include/CrystalGraphics/*.h) ----------------

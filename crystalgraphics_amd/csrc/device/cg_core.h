@@ -245,8 +245,10 @@ CG_HD void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
 // the TLAS.  The closest hit equals the reference's unordered walk
 // (bvh.wesl:56-186) except between triangles hit at exactly the same
 // distance.
-template <bool kAnyHit>
-CG_HD HitRec traverse(const DevScene& S, f3 wo, f3 wd, float tmax, const TStack& stk, TravStats& st) {
+// kMode 0: closest hit; 1: any hit; 2: any hit where `any` is set, per lane
+// (one launch serves a queue that holds rays of both kinds).
+template <int kMode>
+CG_HD HitRec traverse_m(const DevScene& S, f3 wo, f3 wd, float tmax, bool any, const TStack& stk, TravStats& st) {
   HitRec best{tmax, 0.0f, 0.0f, CG_MISS_ID, CG_MISS_ID};
   uint32_t ref = S.root;
   if (ref == CG_REF_NONE) return best;
@@ -315,7 +317,7 @@ CG_HD HitRec traverse(const DevScene& S, f3 wo, f3 wd, float tmax, const TStack&
           best.v = h.v;
           best.inst = tr->inst != CG_MISS_ID ? tr->inst : cur_inst;
           best.slot = slot + i;
-          if (kAnyHit) return best;
+          if (kMode == 1 || (kMode == 2 && any)) return best;
         }
       }
     }
@@ -334,6 +336,11 @@ CG_HD HitRec traverse(const DevScene& S, f3 wo, f3 wd, float tmax, const TStack&
       if (te <= best.t) break;
     }
   }
+}
+
+template <bool kAnyHit>
+CG_HD HitRec traverse(const DevScene& S, f3 wo, f3 wd, float tmax, const TStack& stk, TravStats& st) {
+  return traverse_m<kAnyHit ? 1 : 0>(S, wo, wd, tmax, kAnyHit, stk, st);
 }
 
 // Surface at a hit (bvh.wesl:103-129 and the per-hit attribute fetch of

@@ -16,6 +16,7 @@
 #include "cg_layout.h"
 #include "cg_render.h"
 #include "cg_upload.h"
+#include "cg_wavefront.h"
 
 using namespace cg;
 
@@ -247,6 +248,133 @@ __global__ __launch_bounds__(kBlock) void k_stream_copy(float4* __restrict__ dst
                                                         size_t n) {
   size_t stride = static_cast<size_t>(gridDim.x) * kBlock;
   for (size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront rounds (cg_wavefront.h).  Queues hold slot ids; a wave appends
+// the slots of its lanes that continue with one atomic (ballot, popcount,
+// mbcnt prefix), so 64 neighbouring slots stay neighbours in the next queue.
+__device__ __forceinline__ void wave_push(uint32_t* queue, uint32_t* len, bool push, uint32_t value) {
+  const unsigned long long m = __ballot(push);
+  if (m == 0ull) return;
+  uint32_t base = 0;
+  if ((threadIdx.x & 63u) == 0u) base = atomicAdd(len, static_cast<uint32_t>(__popcll(m)));
+  base = __shfl(base, 0);
+  const uint32_t below =
+      __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+  if (push) queue[base + below] = value;
+}
+
+// Admission: initialise the slots of job j (tile-major) and queue them on
+// queue 0.  One lane per slot; lanes past the picture only vote.
+__global__ __launch_bounds__(kBlock) void k_wf_admit(WfTables T, uint32_t j, uint32_t nslots) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  const WfJob& J = T.jobs[j];
+  const uint32_t slot = J.slot_base + t;
+  bool push = false;
+  if (t < nslots) {
+    uint32_t px, py;
+    slot_pixel(J, t, px, py);
+    if (px < J.W && py < J.H) {
+      T.slot_job[slot] = j;
+      const uint32_t hidx = px + py * J.W;
+      WfState s;
+      s.hash = 2166136261u;
+      s.acc = mk3(0.0f, 0.0f, 0.0f);
+      s.sample = 0u;
+      s.queries = 0u;
+      s.rng = J.iter > 0u ? J.history[hidx].rand_seed : hidx;
+      rng_step(s.rng);
+      if (J.spp > 0u) {
+        wf_start_sample(T, J, px, py, s);
+        wf_store(T.path[slot], s);
+        push = true;
+      } else {
+        wf_finish(J, px, py, s.acc, s.rng, s.hash, 0u);
+        atomicSub(T.job_left + j, 1u);
+      }
+    }
+  }
+  wave_push(T.queue[0], T.counters + kWfQueueLen, push, slot);
+}
+
+// Trace the ray in flight of every slot on queue q.
+__global__ __launch_bounds__(kBlock) void k_wf_trace(WfTables T, uint32_t q, StackArgs sa) {
+  __shared__ uint32_t s_ref[kShortStack * kBlock];
+  __shared__ float s_t[kShortStack * kBlock];
+  const TStack stk = lane_stack(s_ref, s_t, sa);
+  const uint32_t n = T.counters[kWfQueueLen + 32u * q];
+  const WorkQueue wq{T.counters + kWfTraceWork, (n + 63u) / 64u};
+  const uint32_t* queue = T.queue[q];
+  TravStats st{0u, 0u, 0u};
+  uint64_t nray = 0, nshadow = 0;
+  uint32_t probe = 0;
+  for (;;) {
+    const uint32_t item = grab_item(wq, probe);
+    if (item == kNoItem) break;
+    const uint32_t i = item * 64u + (threadIdx.x & 63u);
+    if (i >= n) continue;
+    const uint32_t slot = queue[i];
+    const DevScene& S = T.scenes[T.jobs[T.slot_job[slot]].scene];
+    const float4* rp = reinterpret_cast<const float4*>(T.path + slot);
+    const float4 a = rp[0], b = rp[1];
+    const bool shadow = (__float_as_uint(b.w) & 1u) != 0u;
+    HitRec h = traverse_m<2>(S, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, shadow, stk, st);
+    if (shadow)
+      ++nshadow;
+    else
+      ++nray;
+    float4* hp = reinterpret_cast<float4*>(T.hits + slot);
+    hp[0] = make_float4(h.t, h.u, h.v, __uint_as_float(h.inst));
+    hp[1] = make_float4(__uint_as_float(h.slot), 0.0f, 0.0f, 0.0f);
+  }
+  if (T.stats) flush_stats(T.stats, nray, nshadow, st.nodes, st.tris, st.insts);
+}
+
+// Advance every slot on queue q by one event; continuing slots go to q ^ 1.
+__global__ __launch_bounds__(kBlock, 4) void k_wf_shade(WfTables T, uint32_t q, StackArgs sa) {
+  __shared__ uint32_t s_ref[kShortStack * kBlock];
+  __shared__ float s_t[kShortStack * kBlock];
+  const TStack stk = lane_stack(s_ref, s_t, sa);
+  const uint32_t n = T.counters[kWfQueueLen + 32u * q];
+  const WorkQueue wq{T.counters + kWfShadeWork, (n + 63u) / 64u};
+  const uint32_t* queue = T.queue[q];
+  uint32_t probe = 0;
+  for (;;) {
+    const uint32_t item = grab_item(wq, probe);
+    if (item == kNoItem) break;
+    const uint32_t i = item * 64u + (threadIdx.x & 63u);
+    bool push = false;
+    uint32_t slot = 0u;
+    if (i < n) {
+      slot = queue[i];
+      const uint32_t j = T.slot_job[slot];
+      const WfJob& J = T.jobs[j];
+      const DevScene& S = T.scenes[J.scene];
+      uint32_t px, py;
+      slot_pixel(J, slot - J.slot_base, px, py);
+      WfState s = wf_load(T.path[slot]);
+      const HitW hw = T.hits[slot];
+      push = wf_step(T, J, S, slot, px, py, s, hw, stk);
+      if (push)
+        wf_store(T.path[slot], s);
+      else
+        atomicSub(T.job_left + j, 1u);
+    }
+    wave_push(T.queue[q ^ 1u], T.counters + kWfQueueLen + 32u * (q ^ 1u), push, slot);
+  }
+}
+
+// End of a round: queue q is consumed; clear its length and the work
+// counters of the next round's kernels (a kernel, not a memset, so the round
+// can sit in a captured graph).
+__global__ void k_wf_flip(WfTables T, uint32_t q) {
+  const uint32_t t = threadIdx.x;
+  if (t == 0u) T.counters[kWfQueueLen + 32u * q] = 0u;
+  if (t < kXcds) {
+    T.counters[kWfTraceWork + kCounterStride * t] = 0u;
+    T.counters[kWfShadeWork + kCounterStride * t] = 0u;
+  }
 }
 
 template <typename T>
@@ -664,6 +792,342 @@ int cg_memcpy_d2h(cg_env* env, void* dst, const void* src, size_t bytes) {
   if (!bytes) return CG_OK;
   CG_HIP(env, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, env->stream));
   CG_HIP(env, hipStreamSynchronize(env->stream));
+  return CG_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched rendering: the wavefront engine.
+struct cg_batch {
+  cg_env* env = nullptr;
+  cg_batch_conf conf{};
+  uint32_t frames_per_slot = 0;
+  DevBuf path, hits, frames, slot_job, q0, q1, counters, job_left, jobs, scenes, stats, ovf;
+  WfTables T{};
+  StackArgs sa{};
+  uint32_t grid_trace = 0, grid_shade = 0;
+  struct Job {
+    bool active = false;
+    uint32_t slot_base = 0, nslots = 0;
+  };
+  std::vector<Job> job;
+  std::vector<WfJob> job_dev;        // host images of the device job table
+  std::vector<uint32_t> left_init;   // host images of job_left writes
+  std::vector<uint32_t> left_read;
+  uint32_t qlen_read[64];
+  std::vector<const cg_scene*> scene_ptr;
+  std::vector<DevScene> scene_dev;
+  std::vector<std::pair<uint32_t, uint32_t>> free_list;  // (first slot, count), sorted
+  hipGraphExec_t exec = nullptr;
+  uint64_t bytes = 0;
+  uint32_t active = 0;
+};
+
+namespace {
+
+constexpr uint32_t kBatchMaxScenes = 64u;
+
+void batch_release(cg_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  b->exec = nullptr;
+  for (DevBuf* d : {&b->path, &b->hits, &b->frames, &b->slot_job, &b->q0, &b->q1, &b->counters, &b->job_left,
+                    &b->jobs, &b->scenes, &b->stats, &b->ovf})
+    dev_free(*d);
+}
+
+bool slots_alloc(cg_batch* b, uint32_t n, uint32_t& first) {
+  for (size_t i = 0; i < b->free_list.size(); ++i) {
+    auto& r = b->free_list[i];
+    if (r.second < n) continue;
+    first = r.first;
+    r.first += n;
+    r.second -= n;
+    if (r.second == 0) b->free_list.erase(b->free_list.begin() + static_cast<long>(i));
+    return true;
+  }
+  return false;
+}
+void slots_free(cg_batch* b, uint32_t first, uint32_t n) {
+  auto& fl = b->free_list;
+  auto it = std::lower_bound(fl.begin(), fl.end(), std::make_pair(first, 0u));
+  it = fl.insert(it, std::make_pair(first, n));
+  size_t i = static_cast<size_t>(it - fl.begin());
+  if (i + 1 < fl.size() && fl[i].first + fl[i].second == fl[i + 1].first) {
+    fl[i].second += fl[i + 1].second;
+    fl.erase(fl.begin() + static_cast<long>(i) + 1);
+  }
+  if (i > 0 && fl[i - 1].first + fl[i - 1].second == fl[i].first) {
+    fl[i - 1].second += fl[i].second;
+    fl.erase(fl.begin() + static_cast<long>(i));
+  }
+}
+
+void enqueue_round(cg_batch* b, uint32_t q) {
+  hipStream_t st = b->env->stream;
+  hipLaunchKernelGGL(k_wf_trace, dim3(b->grid_trace), dim3(kBlock), 0, st, b->T, q, b->sa);
+  hipLaunchKernelGGL(k_wf_shade, dim3(b->grid_shade), dim3(kBlock), 0, st, b->T, q, b->sa);
+  hipLaunchKernelGGL(k_wf_flip, dim3(1), dim3(64), 0, st, b->T, q);
+}
+
+int batch_build_graph(cg_batch* b) {
+  cg_env* env = b->env;
+  hipGraph_t g = nullptr;
+  CG_HIP(env, hipStreamBeginCapture(env->stream, hipStreamCaptureModeThreadLocal));
+  for (uint32_t r = 0; r < b->conf.rounds_per_launch; ++r) enqueue_round(b, r & 1u);
+  hipError_t e = hipStreamEndCapture(env->stream, &g);
+  if (e != hipSuccess) return fail(env, CG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(env, CG_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+  return CG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cg_batch_create(cg_env* env, const cg_batch_conf* conf, cg_batch** out) {
+  if (!env || !conf || !out) return fail(env, CG_ERR_INVALID, "cg_batch_create: null argument");
+  *out = nullptr;
+  cg_batch_conf c = *conf;
+  if (c.slot_capacity == 0 || c.slot_capacity > (1u << 28))
+    return fail(env, CG_ERR_INVALID, "slot_capacity must be in [1, 2^28]");
+  c.slot_capacity = (c.slot_capacity + 63u) & ~63u;
+  if (c.max_trace_depth < 1 || c.max_trace_depth > 64) return fail(env, CG_ERR_INVALID, "max_trace_depth must be in [1, 64]");
+  if (c.rounds_per_launch < 2 || (c.rounds_per_launch & 1u))
+    return fail(env, CG_ERR_INVALID, "rounds_per_launch must be even and >= 2");
+  if (c.max_jobs == 0 || c.max_jobs > 4096) return fail(env, CG_ERR_INVALID, "max_jobs must be in [1, 4096]");
+  if (c.max_stack_depth == 0 || c.max_stack_depth > 4096)
+    return fail(env, CG_ERR_INVALID, "max_stack_depth must be in [1, 4096]");
+  cg_batch* b = new cg_batch();
+  b->env = env;
+  b->conf = c;
+  b->frames_per_slot = std::max(1u, c.max_trace_depth - 1u);
+  const size_t S = c.slot_capacity;
+  int rc = CG_OK;
+  auto bail = [&](int code) {
+    batch_release(b);
+    delete b;
+    return code;
+  };
+  auto alloc = [&](DevBuf& d, size_t bytes) {
+    if (rc == CG_OK) rc = dev_alloc(env, d, bytes);
+    b->bytes += bytes;
+  };
+  alloc(b->path, S * sizeof(PathRec));
+  alloc(b->hits, S * sizeof(HitW));
+  alloc(b->frames, S * b->frames_per_slot * sizeof(Frame5));
+  alloc(b->slot_job, S * sizeof(uint32_t));
+  alloc(b->q0, S * sizeof(uint32_t));
+  alloc(b->q1, S * sizeof(uint32_t));
+  alloc(b->counters, kWfCounterWords * sizeof(uint32_t));
+  alloc(b->job_left, c.max_jobs * sizeof(uint32_t));
+  alloc(b->jobs, c.max_jobs * sizeof(WfJob));
+  alloc(b->scenes, kBatchMaxScenes * sizeof(DevScene));
+  alloc(b->stats, 8 * sizeof(unsigned long long));
+  if (rc != CG_OK) return bail(rc);
+  b->grid_trace = resident_grid(k_wf_trace, env, 1u << 30);
+  b->grid_shade = resident_grid(k_wf_shade, env, 1u << 30);
+  // The traversal-stack overflow area is the batch's own (its address is
+  // baked into the graph), sized for the larger of the two resident grids.
+  b->sa.ovf = nullptr;
+  b->sa.ovf_cap = 0;
+  if (c.max_stack_depth > static_cast<uint32_t>(kShortStack)) {
+    b->sa.ovf_cap = static_cast<int>(c.max_stack_depth) - kShortStack;
+    alloc(b->ovf, static_cast<size_t>(std::max(b->grid_trace, b->grid_shade)) * kBlock *
+                      static_cast<size_t>(b->sa.ovf_cap) * sizeof(uint2));
+    if (rc != CG_OK) return bail(rc);
+    b->sa.ovf = static_cast<uint2*>(b->ovf.p);
+  }
+  if (hipMemsetAsync(b->counters.p, 0, b->counters.bytes, env->stream) != hipSuccess ||
+      hipMemsetAsync(b->job_left.p, 0, b->job_left.bytes, env->stream) != hipSuccess ||
+      hipMemsetAsync(b->stats.p, 0, b->stats.bytes, env->stream) != hipSuccess)
+    return bail(fail(env, CG_ERR_HIP, "hipMemsetAsync failed"));
+  WfTables& T = b->T;
+  T.path = static_cast<PathRec*>(b->path.p);
+  T.hits = static_cast<HitW*>(b->hits.p);
+  T.frames = static_cast<Frame5*>(b->frames.p);
+  T.slot_job = static_cast<uint32_t*>(b->slot_job.p);
+  T.queue[0] = static_cast<uint32_t*>(b->q0.p);
+  T.queue[1] = static_cast<uint32_t*>(b->q1.p);
+  T.counters = static_cast<uint32_t*>(b->counters.p);
+  T.job_left = static_cast<uint32_t*>(b->job_left.p);
+  T.jobs = static_cast<const WfJob*>(b->jobs.p);
+  T.scenes = static_cast<const DevScene*>(b->scenes.p);
+  T.wl_cdf = static_cast<const float*>(env->wl_cdf.p);
+  T.wl_leaf = static_cast<const WlLeaf*>(env->wl_leaf.p);
+  T.wl_avg = env->wl_avg;
+  T.slot_cap = c.slot_capacity;
+  T.stats = static_cast<unsigned long long*>(b->stats.p);
+  b->job.resize(c.max_jobs);
+  b->job_dev.resize(c.max_jobs);
+  b->left_init.assign(c.max_jobs, 0u);
+  b->left_read.assign(c.max_jobs, 0u);
+  b->scene_dev.resize(kBatchMaxScenes);
+  b->free_list.push_back({0u, c.slot_capacity});
+  if (c.use_graph && (rc = batch_build_graph(b)) != CG_OK) return bail(rc);
+  if (hipStreamSynchronize(env->stream) != hipSuccess) return bail(fail(env, CG_ERR_HIP, "stream sync failed"));
+  *out = b;
+  return CG_OK;
+}
+
+int cg_batch_destroy(cg_batch* b) {
+  if (!b) return fail(nullptr, CG_ERR_INVALID, "null batch");
+  (void)hipStreamSynchronize(b->env->stream);
+  batch_release(b);
+  delete b;
+  return CG_OK;
+}
+
+int cg_batch_footprint(const cg_batch* b, uint64_t* bytes, uint32_t* slots_free) {
+  if (!b) return fail(nullptr, CG_ERR_INVALID, "null batch");
+  if (bytes) *bytes = b->bytes;
+  if (slots_free) {
+    uint32_t f = 0;
+    for (const auto& r : b->free_list) f = std::max(f, r.second);
+    *slots_free = f;
+  }
+  return CG_OK;
+}
+
+int cg_batch_submit(cg_batch* b, const cg_job* job, uint32_t* job_id) {
+  if (!b || !job || !job->scene || !job->history) return fail(b ? b->env : nullptr, CG_ERR_INVALID, "cg_batch_submit: null argument");
+  cg_env* env = b->env;
+  const cg_conf& c = job->conf;
+  if (c.render_width == 0 || c.render_height == 0) return fail(env, CG_ERR_INVALID, "job render size must be positive");
+  const uint32_t depth = std::min(std::max(c.trace_depth, 1u), 64u);
+  if (depth > b->conf.max_trace_depth)
+    return fail(env, CG_ERR_INVALID, "job trace_depth %u exceeds the batch's %u", depth, b->conf.max_trace_depth);
+  if (std::find(env->scenes.begin(), env->scenes.end(), job->scene) == env->scenes.end())
+    return fail(env, CG_ERR_INVALID, "job scene does not belong to this env");
+  if (job->scene->stack_depth > b->conf.max_stack_depth)
+    return fail(env, CG_ERR_INVALID, "scene needs a %u-entry stack; the batch supports %u", job->scene->stack_depth,
+                b->conf.max_stack_depth);
+  const uint32_t tiles_x = (c.render_width + 7u) / 8u, tiles_y = (c.render_height + 7u) / 8u;
+  const uint64_t nslots = static_cast<uint64_t>(tiles_x) * tiles_y * 64u;
+  if (nslots > b->conf.slot_capacity)
+    return fail(env, CG_ERR_INVALID, "job needs %llu slots; the batch has %u", static_cast<unsigned long long>(nslots),
+                b->conf.slot_capacity);
+  uint32_t sc = 0;
+  while (sc < b->scene_ptr.size() && b->scene_ptr[sc] != job->scene) ++sc;
+  if (sc == b->scene_ptr.size() && sc == kBatchMaxScenes) return fail(env, CG_ERR_FULL, "scene table full");
+  // (re)write the entry when new, or when a released scene's address came back
+  // for a new upload
+  if (sc == b->scene_ptr.size() || std::memcmp(&b->scene_dev[sc], &job->scene->view, sizeof(DevScene)) != 0) {
+    if (sc == b->scene_ptr.size()) b->scene_ptr.push_back(job->scene);
+    b->scene_dev[sc] = job->scene->view;
+    CG_HIP(env, hipMemcpyAsync(static_cast<DevScene*>(b->scenes.p) + sc, &b->scene_dev[sc], sizeof(DevScene),
+                               hipMemcpyHostToDevice, env->stream));
+  }
+  uint32_t j = 0;
+  while (j < b->job.size() && b->job[j].active) ++j;
+  if (j == b->job.size()) return fail(env, CG_ERR_FULL, "job table full");
+  uint32_t first = 0;
+  if (!slots_alloc(b, static_cast<uint32_t>(nslots), first)) return fail(env, CG_ERR_FULL, "no room for %llu slots now",
+                                                                         static_cast<unsigned long long>(nslots));
+  WfJob& J = b->job_dev[j];
+  J = WfJob{};
+  J.scene = sc;
+  J.W = c.render_width;
+  J.H = c.render_height;
+  J.spp = c.sample_count;
+  J.iter = job->iter_count;
+  J.tiles_x = tiles_x;
+  J.slot_base = first;
+  J.P.D = static_cast<int>(depth);
+  J.P.T = c.max_transport_sample_count;
+  J.P.E = c.max_emission_sample_count;
+  J.P.DIF = c.max_diffuse_sample_count;
+  J.P.ROUGH = c.max_rough_sample_count;
+  for (int k = 0; k < 3; ++k) {
+    J.cam_pos[k] = job->camera.position[k];
+    J.cam_dir[k] = job->camera.direction[k];
+  }
+  J.viewport[0] = job->camera.viewport[0];
+  J.viewport[1] = job->camera.viewport[1];
+  J.history = job->history;
+  J.rgba = reinterpret_cast<uint32_t*>(job->rgba8);
+  J.path_hash = job->path_hash;
+  J.ray_count = job->ray_count;
+  b->left_init[j] = c.render_width * c.render_height;
+  CG_HIP(env, hipMemcpyAsync(static_cast<WfJob*>(b->jobs.p) + j, &J, sizeof(WfJob), hipMemcpyHostToDevice, env->stream));
+  CG_HIP(env, hipMemcpyAsync(static_cast<uint32_t*>(b->job_left.p) + j, &b->left_init[j], sizeof(uint32_t),
+                             hipMemcpyHostToDevice, env->stream));
+  hipLaunchKernelGGL(k_wf_admit, dim3(static_cast<uint32_t>((nslots + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                     env->stream, b->T, j, static_cast<uint32_t>(nslots));
+  CG_HIP(env, hipGetLastError());
+  b->job[j].active = true;
+  b->job[j].slot_base = first;
+  b->job[j].nslots = static_cast<uint32_t>(nslots);
+  ++b->active;
+  if (job_id) *job_id = j;
+  return CG_OK;
+}
+
+int cg_batch_launch(cg_batch* b, uint32_t launches) {
+  if (!b) return fail(nullptr, CG_ERR_INVALID, "null batch");
+  cg_env* env = b->env;
+  for (uint32_t l = 0; l < launches; ++l) {
+    if (b->exec) {
+      CG_HIP(env, hipGraphLaunch(b->exec, env->stream));
+    } else {
+      for (uint32_t r = 0; r < b->conf.rounds_per_launch; ++r) enqueue_round(b, r & 1u);
+      CG_HIP(env, hipGetLastError());
+    }
+  }
+  return CG_OK;
+}
+
+int cg_batch_poll(cg_batch* b, uint32_t* done, uint32_t cap, uint32_t* n_done, uint32_t* n_active) {
+  if (!b) return fail(nullptr, CG_ERR_INVALID, "null batch");
+  cg_env* env = b->env;
+  CG_HIP(env, hipMemcpyAsync(b->left_read.data(), b->job_left.p, b->left_read.size() * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, env->stream));
+  CG_HIP(env, hipMemcpyAsync(b->qlen_read, b->counters.p, sizeof(b->qlen_read), hipMemcpyDeviceToHost, env->stream));
+  CG_HIP(env, hipStreamSynchronize(env->stream));
+  uint32_t nd = 0;
+  for (uint32_t j = 0; j < b->job.size(); ++j) {
+    if (!b->job[j].active || b->left_read[j] != 0u) continue;
+    if (done && nd < cap) {
+      done[nd++] = j;
+    } else if (done) {
+      break;  // report the rest next time
+    }
+    b->job[j].active = false;
+    slots_free(b, b->job[j].slot_base, b->job[j].nslots);
+    --b->active;
+  }
+  if (n_done) *n_done = nd;
+  if (n_active) *n_active = b->active;
+  if (b->active > 0 && b->qlen_read[kWfQueueLen] == 0u && b->qlen_read[kWfQueueLen + 32u] == 0u)
+    return fail(env, CG_ERR_STATE, "batch: %u jobs active but no rays queued", b->active);
+  return CG_OK;
+}
+
+int cg_batch_drain(cg_batch* b) {
+  if (!b) return fail(nullptr, CG_ERR_INVALID, "null batch");
+  uint32_t n_active = b->active;
+  while (n_active > 0) {
+    int rc = cg_batch_launch(b, 1);
+    if (rc != CG_OK) return rc;
+    rc = cg_batch_poll(b, nullptr, 0, nullptr, &n_active);
+    if (rc != CG_OK) return rc;
+  }
+  return CG_OK;
+}
+
+int cg_batch_stats(cg_batch* b, cg_stats* out, int reset) {
+  if (!b || !out) return fail(b ? b->env : nullptr, CG_ERR_INVALID, "null argument");
+  unsigned long long v[8];
+  CG_HIP(b->env, hipMemcpyAsync(v, b->stats.p, sizeof(v), hipMemcpyDeviceToHost, b->env->stream));
+  if (reset) CG_HIP(b->env, hipMemsetAsync(b->stats.p, 0, b->stats.bytes, b->env->stream));
+  CG_HIP(b->env, hipStreamSynchronize(b->env->stream));
+  out->rays = v[0];
+  out->shadow_rays = v[1];
+  out->nodes = v[2];
+  out->tris = v[3];
+  out->insts = v[4];
   return CG_OK;
 }
 

@@ -98,6 +98,19 @@ class SceneDescC(C.Structure):
                 ("tag", C.c_uint64)]
 
 
+class BatchConf(C.Structure):
+    """cg_batch_conf"""
+    _fields_ = [("slot_capacity", C.c_uint32), ("max_trace_depth", C.c_uint32), ("rounds_per_launch", C.c_uint32),
+                ("use_graph", C.c_uint32), ("max_jobs", C.c_uint32), ("max_stack_depth", C.c_uint32)]
+
+
+class Job(C.Structure):
+    """cg_job"""
+    _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("conf", Conf), ("iter_count", C.c_uint32),
+                ("_pad", C.c_uint32), ("history", C.c_void_p), ("rgba8", C.c_void_p), ("path_hash", C.c_void_p),
+                ("ray_count", C.c_void_p)]
+
+
 class Stats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("nodes", C.c_uint64),
                 ("tris", C.c_uint64), ("insts", C.c_uint64)]
@@ -196,6 +209,14 @@ def device() -> C.CDLL:
         _sig(d, "cg_device_free", I32, P, P)
         _sig(d, "cg_memcpy_h2d", I32, P, P, P, C.c_size_t)
         _sig(d, "cg_memcpy_d2h", I32, P, P, P, C.c_size_t)
+        _sig(d, "cg_batch_create", I32, P, C.POINTER(BatchConf), C.POINTER(P))
+        _sig(d, "cg_batch_destroy", I32, P)
+        _sig(d, "cg_batch_footprint", I32, P, C.POINTER(C.c_uint64), C.POINTER(U32))
+        _sig(d, "cg_batch_submit", I32, P, C.POINTER(Job), C.POINTER(U32))
+        _sig(d, "cg_batch_launch", I32, P, U32)
+        _sig(d, "cg_batch_poll", I32, P, P, U32, C.POINTER(U32), C.POINTER(U32))
+        _sig(d, "cg_batch_drain", I32, P)
+        _sig(d, "cg_batch_stats", I32, P, C.POINTER(Stats), I32)
         _dev = d
     return _dev
 

@@ -33,15 +33,17 @@ enum {
   CG_ERR_HIP = 2,          /* HIP runtime error */
   CG_ERR_OUT_OF_MEMORY = 3,
   CG_ERR_UNSUPPORTED = 4,  /* scene feature the kernels do not implement */
-  CG_ERR_STATE = 5         /* call out of order (e.g. view before upload) */
+  CG_ERR_STATE = 5,        /* call out of order (e.g. view before upload) */
+  CG_ERR_FULL = 6          /* no room now (batch slots or tables); retry later */
 };
 
 typedef struct cg_env cg_env;
 typedef struct cg_scene cg_scene;
 
 /* Traversal counters accumulated by cg_view / cg_trace_* while counting is
- * enabled: closest-hit and any-hit queries, 64-B pair nodes fetched, 48-B
- * triangle records tested, 80-B instance records entered. */
+ * enabled (and always by a batch): closest-hit and any-hit queries, 128-B
+ * 4-wide BVH nodes visited, 48-B triangle records tested, 80-B instance
+ * records entered. */
 typedef struct cg_stats {
   uint64_t rays;
   uint64_t shadow_rays;
@@ -125,6 +127,55 @@ int cg_device_alloc(cg_env* env, size_t bytes, void** dptr);
 int cg_device_free(cg_env* env, void* dptr);
 int cg_memcpy_h2d(cg_env* env, void* dst, const void* src, size_t bytes);  /* synchronises */
 int cg_memcpy_d2h(cg_env* env, void* dst, const void* src, size_t bytes);  /* synchronises */
+
+/* ---------------------------------------------------------------------------
+ * Batched rendering (a form the reference does not have: it renders one
+ * View at a time, src/wgpu/view.cpp:16-88).  A job is that View's work -- one
+ * progressive frame of one scene and camera -- at its own size, spp, depth
+ * and lobe caps, with its own history.  A batch keeps a pool of pixel slots
+ * whose path state lives in HBM and advances every admitted job together in
+ * rounds (trace the queued rays, advance each path one event, queue its next
+ * ray), `rounds_per_launch` rounds per launch, captured once as a hipGraph.
+ * Jobs are admitted between launches (continuous batching); each pixel's
+ * result equals cg_view's for the same scene, camera, conf and history, bit
+ * for bit (the same per-pixel operations in the same order).
+ * Scenes must stay uploaded while a job on them is active. */
+typedef struct cg_batch cg_batch;
+typedef struct cg_batch_conf {
+  uint32_t slot_capacity;      /* pixels resident at once (rounded up to 64) */
+  uint32_t max_trace_depth;    /* deepest job accepted (1-64) */
+  uint32_t rounds_per_launch;  /* even, >= 2 */
+  uint32_t use_graph;          /* 1: one launch = one replay of a captured hipGraph */
+  uint32_t max_jobs;           /* job table entries (1-4096) */
+  uint32_t max_stack_depth;    /* deepest traversal stack of an accepted scene */
+} cg_batch_conf;
+typedef struct cg_job {
+  cg_scene* scene;
+  cg_camera camera;
+  cg_conf conf;                /* render size, sample_count, trace_depth, lobe caps */
+  uint32_t iter_count;         /* 0 starts a new accumulation */
+  uint32_t _pad;
+  cg_history_pixel* history;   /* device, W*H: read when iter_count > 0, written */
+  uint8_t* rgba8;              /* device, W*H*4, nullable */
+  uint32_t* path_hash;         /* device, W*H, nullable (path records) */
+  uint32_t* ray_count;         /* device, W*H, nullable */
+} cg_job;
+int cg_batch_create(cg_env* env, const cg_batch_conf* conf, cg_batch** out);
+int cg_batch_destroy(cg_batch* batch);
+/* Device bytes held, and the largest run of free slots (what fits now). */
+int cg_batch_footprint(const cg_batch* batch, uint64_t* bytes, uint32_t* slots_free);
+/* Admit a job: CG_ERR_FULL when its (ceil(W/8)*ceil(H/8)*64) slots or a
+ * table entry are not free now. */
+int cg_batch_submit(cg_batch* batch, const cg_job* job, uint32_t* job_id);
+/* Enqueue `launches` launches of rounds (asynchronous). */
+int cg_batch_launch(cg_batch* batch, uint32_t launches);
+/* Jobs finished since the last poll (ids into `done`, up to cap), and the
+ * number still active.  Synchronises. */
+int cg_batch_poll(cg_batch* batch, uint32_t* done, uint32_t cap, uint32_t* n_done, uint32_t* n_active);
+/* Launch and poll until no job is active.  Synchronises. */
+int cg_batch_drain(cg_batch* batch);
+/* Traversal counters of the batch's rounds so far (optionally reset). */
+int cg_batch_stats(cg_batch* batch, cg_stats* out, int reset);
 
 #ifdef __cplusplus
 }
