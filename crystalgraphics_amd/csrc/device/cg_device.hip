@@ -116,8 +116,11 @@ __device__ __forceinline__ uint32_t grab_item(const WorkQueue& q, uint32_t& prob
       const uint32_t x = (home + probe) % kXcds;
       const uint32_t begin = x * per;
       const uint32_t end = begin + per < q.total ? begin + per : q.total;
-      if (begin < end) {
-        const uint32_t c = atomicAdd(q.counters + x * kCounterStride, 1u);
+      uint32_t* ctr = q.counters + x * kCounterStride;
+      // the counter only grows: a plain coherent load rules out an exhausted
+      // range without a read-modify-write that would queue on its address
+      if (begin < end && begin + __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < end) {
+        const uint32_t c = atomicAdd(ctr, 1u);
         if (begin + c < end) {
           item = begin + c;
           break;
@@ -138,6 +141,10 @@ __device__ __forceinline__ uint64_t wave_sum(uint64_t v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
 }
+// Traversal counters: kStatCopies copies of 8 words, one chosen per wave, so
+// the adds of thousands of waves do not queue on one address; readers sum.
+constexpr uint32_t kStatCopies = 64u;
+constexpr size_t kStatBytes = kStatCopies * 8 * sizeof(unsigned long long);
 __device__ __forceinline__ void flush_stats(unsigned long long* stats, uint64_t rays, uint64_t shadows,
                                             uint64_t nodes, uint64_t tris, uint64_t insts) {
   rays = wave_sum(rays);
@@ -145,12 +152,14 @@ __device__ __forceinline__ void flush_stats(unsigned long long* stats, uint64_t 
   nodes = wave_sum(nodes);
   tris = wave_sum(tris);
   insts = wave_sum(insts);
+  if ((rays | shadows | nodes | tris | insts) == 0u) return;
   if ((threadIdx.x & 63u) == 0u) {
-    atomicAdd(stats + 0, static_cast<unsigned long long>(rays));
-    atomicAdd(stats + 1, static_cast<unsigned long long>(shadows));
-    atomicAdd(stats + 2, static_cast<unsigned long long>(nodes));
-    atomicAdd(stats + 3, static_cast<unsigned long long>(tris));
-    atomicAdd(stats + 4, static_cast<unsigned long long>(insts));
+    unsigned long long* c = stats + 8u * ((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % kStatCopies);
+    if (rays) atomicAdd(c + 0, static_cast<unsigned long long>(rays));
+    if (shadows) atomicAdd(c + 1, static_cast<unsigned long long>(shadows));
+    if (nodes) atomicAdd(c + 2, static_cast<unsigned long long>(nodes));
+    if (tris) atomicAdd(c + 3, static_cast<unsigned long long>(tris));
+    if (insts) atomicAdd(c + 4, static_cast<unsigned long long>(insts));
   }
 }
 
@@ -496,7 +505,7 @@ int cg_env_create(const cg_conf* conf, int device, cg_env** out) {
   const size_t px = static_cast<size_t>(conf->render_width) * conf->render_height;
   if ((rc = dev_alloc(env, env->history, px * sizeof(cg_history_pixel))) != CG_OK) return bail(rc);
   if ((rc = dev_alloc(env, env->rgba, px * 4)) != CG_OK) return bail(rc);
-  if ((rc = dev_alloc(env, env->stats, 8 * sizeof(unsigned long long))) != CG_OK) return bail(rc);
+  if ((rc = dev_alloc(env, env->stats, kStatBytes)) != CG_OK) return bail(rc);
   if ((rc = dev_alloc(env, env->queue, kXcds * kCounterStride * sizeof(uint32_t))) != CG_OK) return bail(rc);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return bail(fail(env, CG_ERR_HIP, "hipGetDeviceProperties failed"));
@@ -571,16 +580,26 @@ int cg_env_set_accel(cg_env* env, int mode) {
   return CG_OK;
 }
 
+namespace {
+void sum_stats(const unsigned long long* v, cg_stats* out) {
+  *out = cg_stats{0, 0, 0, 0, 0};
+  for (uint32_t c = 0; c < kStatCopies; ++c) {
+    const unsigned long long* w = v + 8u * c;
+    out->rays += w[0];
+    out->shadow_rays += w[1];
+    out->nodes += w[2];
+    out->tris += w[3];
+    out->insts += w[4];
+  }
+}
+}  // namespace
+
 int cg_stats_read(cg_env* env, cg_stats* out) {
   if (!env || !out) return fail(env, CG_ERR_INVALID, "null argument");
-  unsigned long long v[8];
-  CG_HIP(env, hipMemcpyAsync(v, env->stats.p, sizeof(v), hipMemcpyDeviceToHost, env->stream));
+  std::vector<unsigned long long> v(kStatCopies * 8);
+  CG_HIP(env, hipMemcpyAsync(v.data(), env->stats.p, kStatBytes, hipMemcpyDeviceToHost, env->stream));
   CG_HIP(env, hipStreamSynchronize(env->stream));
-  out->rays = v[0];
-  out->shadow_rays = v[1];
-  out->nodes = v[2];
-  out->tris = v[3];
-  out->insts = v[4];
+  sum_stats(v.data(), out);
   return CG_OK;
 }
 
@@ -925,7 +944,7 @@ int cg_batch_create(cg_env* env, const cg_batch_conf* conf, cg_batch** out) {
   alloc(b->job_left, c.max_jobs * sizeof(uint32_t));
   alloc(b->jobs, c.max_jobs * sizeof(WfJob));
   alloc(b->scenes, kBatchMaxScenes * sizeof(DevScene));
-  alloc(b->stats, 8 * sizeof(unsigned long long));
+  alloc(b->stats, kStatBytes);
   if (rc != CG_OK) return bail(rc);
   b->grid_trace = resident_grid(k_wf_trace, env, 1u << 30);
   b->grid_shade = resident_grid(k_wf_shade, env, 1u << 30);
@@ -1119,15 +1138,11 @@ int cg_batch_drain(cg_batch* b) {
 
 int cg_batch_stats(cg_batch* b, cg_stats* out, int reset) {
   if (!b || !out) return fail(b ? b->env : nullptr, CG_ERR_INVALID, "null argument");
-  unsigned long long v[8];
-  CG_HIP(b->env, hipMemcpyAsync(v, b->stats.p, sizeof(v), hipMemcpyDeviceToHost, b->env->stream));
+  std::vector<unsigned long long> v(kStatCopies * 8);
+  CG_HIP(b->env, hipMemcpyAsync(v.data(), b->stats.p, kStatBytes, hipMemcpyDeviceToHost, b->env->stream));
   if (reset) CG_HIP(b->env, hipMemsetAsync(b->stats.p, 0, b->stats.bytes, b->env->stream));
   CG_HIP(b->env, hipStreamSynchronize(b->env->stream));
-  out->rays = v[0];
-  out->shadow_rays = v[1];
-  out->nodes = v[2];
-  out->tris = v[3];
-  out->insts = v[4];
+  sum_stats(v.data(), out);
   return CG_OK;
 }
 
