@@ -353,6 +353,10 @@ class RenderJob:
         j.ray_count = self.ray_count.ptr if self.ray_count else None
         return j
 
+    def view_alone(self) -> None:
+        """Render this job with the plain one-lane-per-pixel kernel (cg_view_job)."""
+        self.env._check(native.device().cg_view_job(self.env.handle, C.byref(self.c())))
+
     def read_history(self) -> np.ndarray:
         return self.history.download(HISTORY, self.npix).reshape(self.conf.render_height, self.conf.render_width)
 

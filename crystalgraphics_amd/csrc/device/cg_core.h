@@ -245,97 +245,147 @@ CG_HD void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
 // the TLAS.  The closest hit equals the reference's unordered walk
 // (bvh.wesl:56-186) except between triangles hit at exactly the same
 // distance.
-// kMode 0: closest hit; 1: any hit; 2: any hit where `any` is set, per lane
-// (one launch serves a queue that holds rays of both kinds).
+// The scene pointers a traversal reads (per lane: lanes of one wave may
+// trace rays of different scenes).
+struct TravScene {
+  const Node4* bvh;
+  const TriRec* tris;
+  const InstRec* insts;
+  uint32_t root;
+};
+CG_HD TravScene trav_scene(const DevScene& S) { return TravScene{S.bvh, S.tris, S.insts, S.root}; }
+
+// A traversal in progress, resumable one visit at a time.
+struct TravRay {
+  f3 wo, wd, winv;  // world-space ray
+  f3 o, d, inv;     // ray in the space of the BLAS being walked
+  HitRec best;
+  uint32_t ref;     // next node, instance or leaf to visit
+  uint32_t cur_inst;
+  int sp;
+  bool any;         // any-hit query: finished at the first hit below tmax
+};
+
+// Start a traversal; false when the scene is empty (the ray misses).
+CG_HD bool trav_begin(const TravScene& S, TravRay& r, f3 wo, f3 wd, float tmax, bool any) {
+  r.best = HitRec{tmax, 0.0f, 0.0f, CG_MISS_ID, CG_MISS_ID};
+  r.ref = S.root;
+  r.wo = wo;
+  r.wd = wd;
+  r.winv = dir_inv(wd);
+  r.o = wo;
+  r.d = wd;
+  r.inv = r.winv;
+  r.cur_inst = CG_MISS_ID;
+  r.sp = 0;
+  r.any = any;
+  return r.ref != CG_REF_NONE;
+}
+
+// Two-level closest-hit (or any-hit) traversal of the 4-wide BVH, one visit
+// per call: at a node the hit children are sorted by entry distance, the
+// nearest is visited next and the others are pushed with their distances;
+// popped entries entered beyond the best hit so far are dropped; a marker on
+// the stack returns from a BLAS to the TLAS.  Returns true when finished
+// (r.best holds the result).  The closest hit equals the reference's
+// unordered walk (bvh.wesl:56-186) except between triangles hit at exactly
+// the same distance.  kMode 0: closest hit; 1: any hit; 2: any hit where
+// r.any is set (one launch serves a queue holding rays of both kinds).
 template <int kMode>
-CG_HD HitRec traverse_m(const DevScene& S, f3 wo, f3 wd, float tmax, bool any, const TStack& stk, TravStats& st) {
-  HitRec best{tmax, 0.0f, 0.0f, CG_MISS_ID, CG_MISS_ID};
-  uint32_t ref = S.root;
-  if (ref == CG_REF_NONE) return best;
+CG_HD bool trav_step(const TravScene& S, TravRay& r, const TStack& stk, TravStats& st) {
   const float kInf = __builtin_inff();
-  const f3 winv = dir_inv(wd);
-  f3 o = wo, d = wd, inv = winv;
-  uint32_t cur_inst = CG_MISS_ID;
-  int sp = 0;
-  for (;;) {
-    if ((ref & (CG_REF_LEAF | CG_REF_INST)) == 0u) {
-      const float4* q = reinterpret_cast<const float4*>(S.bvh + ref);
-      const float4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
-      const uint4 rr = reinterpret_cast<const uint4*>(q)[6];
-      st.nodes++;
-      float t0, t1, t2, t3;
-      uint32_t r0 = rr.x, r1 = rr.y, r2 = rr.z, r3 = rr.w;
-      {
-        const float b0[3] = {lx.x, ly.x, lz.x}, e0[3] = {hx.x, hy.x, hz.x};
-        const float b1[3] = {lx.y, ly.y, lz.y}, e1[3] = {hx.y, hy.y, hz.y};
-        const float b2[3] = {lx.z, ly.z, lz.z}, e2[3] = {hx.z, hy.z, hz.z};
-        const float b3[3] = {lx.w, ly.w, lz.w}, e3[3] = {hx.w, hy.w, hz.w};
-        t0 = box_enter(o, inv, b0, e0);
-        t1 = box_enter(o, inv, b1, e1);
-        t2 = box_enter(o, inv, b2, e2);
-        t3 = box_enter(o, inv, b3, e3);
-      }
-      t0 = (t0 <= best.t && r0 != CG_REF_NONE) ? t0 : kInf;
-      t1 = (t1 <= best.t && r1 != CG_REF_NONE) ? t1 : kInf;
-      t2 = (t2 <= best.t && r2 != CG_REF_NONE) ? t2 : kInf;
-      t3 = (t3 <= best.t && r3 != CG_REF_NONE) ? t3 : kInf;
-      cswap(t0, r0, t1, r1);
-      cswap(t2, r2, t3, r3);
-      cswap(t0, r0, t2, r2);
-      cswap(t1, r1, t3, r3);
-      cswap(t1, r1, t2, r2);
-      if (t3 < kInf) stk.push(sp, r3, t3);
-      if (t2 < kInf) stk.push(sp, r2, t2);
-      if (t1 < kInf) stk.push(sp, r1, t1);
-      if (t0 < kInf) {
-        ref = r0;
-        continue;
-      }
-    } else if ((ref & CG_REF_LEAF) == 0u) {
-      uint32_t ii = ref & CG_REF_INDEX_MASK;
-      const InstRec* in = S.insts + ii;
-      st.insts++;
-      if (in->root != CG_REF_NONE) {
-        o = mat_point(in->inv, wo);
-        d = mat_dir(in->inv, wd);
-        inv = dir_inv(d);
-        cur_inst = ii;
-        stk.push(sp, CG_REF_MARKER, 0.0f);
-        ref = in->root;
-        continue;
-      }
-    } else {
-      const uint32_t slot = ref & CG_REF_SLOT_MASK;
-      const uint32_t cnt = ((ref >> CG_REF_COUNT_SHIFT) & 3u) + 1u;
-      for (uint32_t i = 0; i < cnt; ++i) {
-        const TriRec* tr = S.tris + slot + i;
-        st.tris++;
-        TriT h = tri_test(o, d, ld3(tr->v0), ld3(tr->e1), ld3(tr->e2));
-        if (h.t > 0.0f && h.t < best.t) {
-          best.t = h.t;
-          best.u = h.u;
-          best.v = h.v;
-          best.inst = tr->inst != CG_MISS_ID ? tr->inst : cur_inst;
-          best.slot = slot + i;
-          if (kMode == 1 || (kMode == 2 && any)) return best;
-        }
-      }
+  const uint32_t ref = r.ref;
+  if ((ref & (CG_REF_LEAF | CG_REF_INST)) == 0u) {
+    const float4* q = reinterpret_cast<const float4*>(S.bvh + ref);
+    const float4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
+    const uint4 rr = reinterpret_cast<const uint4*>(q)[6];
+    st.nodes++;
+    float t0, t1, t2, t3;
+    uint32_t r0 = rr.x, r1 = rr.y, r2 = rr.z, r3 = rr.w;
+    {
+      const float b0[3] = {lx.x, ly.x, lz.x}, e0[3] = {hx.x, hy.x, hz.x};
+      const float b1[3] = {lx.y, ly.y, lz.y}, e1[3] = {hx.y, hy.y, hz.y};
+      const float b2[3] = {lx.z, ly.z, lz.z}, e2[3] = {hx.z, hy.z, hz.z};
+      const float b3[3] = {lx.w, ly.w, lz.w}, e3[3] = {hx.w, hy.w, hz.w};
+      t0 = box_enter(r.o, r.inv, b0, e0);
+      t1 = box_enter(r.o, r.inv, b1, e1);
+      t2 = box_enter(r.o, r.inv, b2, e2);
+      t3 = box_enter(r.o, r.inv, b3, e3);
     }
-    // pop
-    for (;;) {
-      if (sp == 0) return best;
-      float te;
-      stk.pop(sp, ref, te);
-      if (ref == CG_REF_MARKER) {
-        o = wo;
-        d = wd;
-        inv = winv;
-        cur_inst = CG_MISS_ID;
-        continue;
+    t0 = (t0 <= r.best.t && r0 != CG_REF_NONE) ? t0 : kInf;
+    t1 = (t1 <= r.best.t && r1 != CG_REF_NONE) ? t1 : kInf;
+    t2 = (t2 <= r.best.t && r2 != CG_REF_NONE) ? t2 : kInf;
+    t3 = (t3 <= r.best.t && r3 != CG_REF_NONE) ? t3 : kInf;
+    cswap(t0, r0, t1, r1);
+    cswap(t2, r2, t3, r3);
+    cswap(t0, r0, t2, r2);
+    cswap(t1, r1, t3, r3);
+    cswap(t1, r1, t2, r2);
+    if (t3 < kInf) stk.push(r.sp, r3, t3);
+    if (t2 < kInf) stk.push(r.sp, r2, t2);
+    if (t1 < kInf) stk.push(r.sp, r1, t1);
+    if (t0 < kInf) {
+      r.ref = r0;
+      return false;
+    }
+  } else if ((ref & CG_REF_LEAF) == 0u) {
+    const uint32_t ii = ref & CG_REF_INDEX_MASK;
+    const InstRec* in = S.insts + ii;
+    st.insts++;
+    if (in->root != CG_REF_NONE) {
+      r.o = mat_point(in->inv, r.wo);
+      r.d = mat_dir(in->inv, r.wd);
+      r.inv = dir_inv(r.d);
+      r.cur_inst = ii;
+      stk.push(r.sp, CG_REF_MARKER, 0.0f);
+      r.ref = in->root;
+      return false;
+    }
+  } else {
+    const uint32_t slot = ref & CG_REF_SLOT_MASK;
+    const uint32_t cnt = ((ref >> CG_REF_COUNT_SHIFT) & 3u) + 1u;
+    for (uint32_t i = 0; i < cnt; ++i) {
+      const TriRec* tr = S.tris + slot + i;
+      st.tris++;
+      TriT h = tri_test(r.o, r.d, ld3(tr->v0), ld3(tr->e1), ld3(tr->e2));
+      if (h.t > 0.0f && h.t < r.best.t) {
+        r.best.t = h.t;
+        r.best.u = h.u;
+        r.best.v = h.v;
+        r.best.inst = tr->inst != CG_MISS_ID ? tr->inst : r.cur_inst;
+        r.best.slot = slot + i;
+        if (kMode == 1 || (kMode == 2 && r.any)) return true;
       }
-      if (te <= best.t) break;
     }
   }
+  // pop
+  for (;;) {
+    if (r.sp == 0) return true;
+    float te;
+    uint32_t nr;
+    stk.pop(r.sp, nr, te);
+    if (nr == CG_REF_MARKER) {
+      r.o = r.wo;
+      r.d = r.wd;
+      r.inv = r.winv;
+      r.cur_inst = CG_MISS_ID;
+      continue;
+    }
+    if (te <= r.best.t) {
+      r.ref = nr;
+      return false;
+    }
+  }
+}
+
+template <int kMode>
+CG_HD HitRec traverse_m(const DevScene& S, f3 wo, f3 wd, float tmax, bool any, const TStack& stk, TravStats& st) {
+  const TravScene ts = trav_scene(S);
+  TravRay r;
+  if (!trav_begin(ts, r, wo, wd, tmax, any)) return r.best;
+  while (!trav_step<kMode>(ts, r, stk, st)) {
+  }
+  return r.best;
 }
 
 template <bool kAnyHit>

@@ -179,8 +179,69 @@ __device__ __forceinline__ TStack lane_stack(uint32_t* s_ref, float* s_t, const 
                 sa.ovf ? sa.ovf_cap : 0};
 }
 
+// Persistent traversal of queued rays with lane refill: a wave takes rays
+// 64 at a time from the work queue and hands them to its idle lanes; each
+// lane advances its own ray one visit per step, and once a quarter of the
+// wave is idle the idle lanes take the next rays.  Lanes stay busy instead of
+// waiting for the slowest ray of the wave.  fetch(i, scene, o, d, tmax, any)
+// sets up ray i; done(i, ray) stores its result.
+constexpr int kRefillLanes = 16;
+constexpr int kStepsPerRefill = 8;
+template <int kMode, class Fetch, class Done>
+__device__ __forceinline__ void trace_refill(const WorkQueue& q, uint32_t n, const TStack& stk, TravStats& st,
+                                             Fetch fetch, Done done) {
+  uint32_t probe = 0;
+  uint32_t cbeg = 0, cend = 0;  // rays of the current chunk not yet handed out (wave-uniform)
+  bool more = true;             // the queue may hold more chunks (wave-uniform)
+  bool has = false;
+  uint32_t my = 0;
+  TravScene ts{};
+  TravRay r;
+  for (;;) {
+    if (more && __popcll(__ballot(!has)) >= kRefillLanes) {
+      for (;;) {
+        const unsigned long long idle = __ballot(!has);
+        if (idle == 0ull) break;
+        if (cbeg == cend) {
+          const uint32_t item = grab_item(q, probe);
+          if (item == kNoItem) {
+            more = false;
+            break;
+          }
+          cbeg = item * 64u;
+          cend = cbeg + 64u < n ? cbeg + 64u : n;
+          continue;
+        }
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(idle >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(idle), 0u));
+        const uint32_t avail = cend - cbeg;
+        if (!has && rank < avail) {
+          my = cbeg + rank;
+          f3 o, d;
+          float tmax;
+          bool any;
+          fetch(my, ts, o, d, tmax, any);
+          if (trav_begin(ts, r, o, d, tmax, any))
+            has = true;
+          else
+            done(my, r);
+        }
+        const uint32_t taken = static_cast<uint32_t>(__popcll(idle));
+        cbeg += avail < taken ? avail : taken;
+      }
+    }
+    if (!more && __ballot(has) == 0ull) break;
+    for (int k = 0; k < kStepsPerRefill; ++k) {
+      if (has && trav_step<kMode>(ts, r, stk, st)) {
+        done(my, r);
+        has = false;
+      }
+    }
+  }
+}
+
 template <bool kAnyHit>
-__global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const cg_ray* __restrict__ rays,
+__global__ __launch_bounds__(kBlock, 5) void k_trace(DevScene S, const cg_ray* __restrict__ rays,
                                                   const float* __restrict__ tmax, uint32_t n,
                                                   cg_hit* __restrict__ hits, float* __restrict__ uv,
                                                   uint32_t* __restrict__ occluded,
@@ -188,41 +249,43 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, const cg_ray* __re
   __shared__ uint32_t s_ref[kShortStack * kBlock];
   __shared__ float s_t[kShortStack * kBlock];
   const TStack stk = lane_stack(s_ref, s_t, sa);
+  const TravScene scene = trav_scene(S);
   TravStats st{0u, 0u, 0u};
   uint64_t nq = 0;
-  uint32_t probe = 0;
-  for (;;) {
-    const uint32_t item = grab_item(q, probe);
-    if (item == kNoItem) break;
-    const uint32_t i = item * 64u + (threadIdx.x & 63u);
-    if (i >= n) continue;
-    ++nq;
+  auto fetch = [&](uint32_t i, TravScene& ts, f3& o, f3& d, float& tm, bool& any) {
     const float4* rp = reinterpret_cast<const float4*>(rays + i);
-    float4 a = rp[0], b = rp[1];
-    f3 o = mk3(a.x, a.y, a.z), d = mk3(b.x, b.y, b.z);
+    const float4 a = rp[0], b = rp[1];
+    o = mk3(a.x, a.y, a.z);
+    d = mk3(b.x, b.y, b.z);
+    tm = kAnyHit ? fmn(tmax[i], kMax) : kMax;
+    any = kAnyHit;
+    ts = scene;
+  };
+  auto done = [&](uint32_t i, const TravRay& r) {
+    ++nq;
+    const HitRec& h = r.best;
     if (kAnyHit) {
-      HitRec h = traverse<true>(S, o, d, fmn(tmax[i], kMax), stk, st);
       occluded[i] = h.inst != CG_MISS_ID ? 1u : 0u;
-    } else {
-      HitRec h = traverse<false>(S, o, d, kMax, stk, st);
-      cg_hit out;
-      if (h.inst != CG_MISS_ID) {
-        out.t = h.t;
-        out.inst = h.inst;
-        out.tri = S.tris[h.slot].tri_id;
-      } else {
-        out.t = CG_MISS_T;
-        out.inst = CG_MISS_ID;
-        out.tri = CG_MISS_ID;
-      }
-      out._pad = 0u;
-      hits[i] = out;
-      if (uv) {
-        uv[2 * i] = h.inst != CG_MISS_ID ? h.u : 0.0f;
-        uv[2 * i + 1] = h.inst != CG_MISS_ID ? h.v : 0.0f;
-      }
+      return;
     }
-  }
+    cg_hit out;
+    if (h.inst != CG_MISS_ID) {
+      out.t = h.t;
+      out.inst = h.inst;
+      out.tri = S.tris[h.slot].tri_id;
+    } else {
+      out.t = CG_MISS_T;
+      out.inst = CG_MISS_ID;
+      out.tri = CG_MISS_ID;
+    }
+    out._pad = 0u;
+    hits[i] = out;
+    if (uv) {
+      uv[2 * i] = h.inst != CG_MISS_ID ? h.u : 0.0f;
+      uv[2 * i + 1] = h.inst != CG_MISS_ID ? h.v : 0.0f;
+    }
+  };
+  trace_refill<kAnyHit ? 1 : 0>(q, n, stk, st, fetch, done);
   if (stats) flush_stats(stats, kAnyHit ? 0u : nq, kAnyHit ? nq : 0u, st.nodes, st.tris, st.insts);
 }
 
@@ -308,7 +371,7 @@ __global__ __launch_bounds__(kBlock) void k_wf_admit(WfTables T, uint32_t j, uin
 }
 
 // Trace the ray in flight of every slot on queue q.
-__global__ __launch_bounds__(kBlock) void k_wf_trace(WfTables T, uint32_t q, StackArgs sa) {
+__global__ __launch_bounds__(kBlock, 5) void k_wf_trace(WfTables T, uint32_t q, StackArgs sa) {
   __shared__ uint32_t s_ref[kShortStack * kBlock];
   __shared__ float s_t[kShortStack * kBlock];
   const TStack stk = lane_stack(s_ref, s_t, sa);
@@ -317,26 +380,28 @@ __global__ __launch_bounds__(kBlock) void k_wf_trace(WfTables T, uint32_t q, Sta
   const uint32_t* queue = T.queue[q];
   TravStats st{0u, 0u, 0u};
   uint64_t nray = 0, nshadow = 0;
-  uint32_t probe = 0;
-  for (;;) {
-    const uint32_t item = grab_item(wq, probe);
-    if (item == kNoItem) break;
-    const uint32_t i = item * 64u + (threadIdx.x & 63u);
-    if (i >= n) continue;
-    const uint32_t slot = queue[i];
-    const DevScene& S = T.scenes[T.jobs[T.slot_job[slot]].scene];
+  uint32_t slot = 0;
+  auto fetch = [&](uint32_t i, TravScene& ts, f3& o, f3& d, float& tm, bool& any) {
+    slot = queue[i];
+    ts = trav_scene(T.scenes[T.jobs[T.slot_job[slot]].scene]);
     const float4* rp = reinterpret_cast<const float4*>(T.path + slot);
     const float4 a = rp[0], b = rp[1];
-    const bool shadow = (__float_as_uint(b.w) & 1u) != 0u;
-    HitRec h = traverse_m<2>(S, mk3(a.x, a.y, a.z), mk3(b.x, b.y, b.z), a.w, shadow, stk, st);
-    if (shadow)
+    o = mk3(a.x, a.y, a.z);
+    d = mk3(b.x, b.y, b.z);
+    tm = a.w;
+    any = (__float_as_uint(b.w) & 1u) != 0u;
+  };
+  auto done = [&](uint32_t, const TravRay& r) {
+    if (r.any)
       ++nshadow;
     else
       ++nray;
+    const HitRec& h = r.best;
     float4* hp = reinterpret_cast<float4*>(T.hits + slot);
     hp[0] = make_float4(h.t, h.u, h.v, __uint_as_float(h.inst));
     hp[1] = make_float4(__uint_as_float(h.slot), 0.0f, 0.0f, 0.0f);
-  }
+  };
+  trace_refill<2>(wq, n, stk, st, fetch, done);
   if (T.stats) flush_stats(T.stats, nray, nshadow, st.nodes, st.tris, st.insts);
 }
 
@@ -717,6 +782,45 @@ int cg_view(cg_env* env, cg_scene* scene, const cg_camera* camera, uint32_t iter
   const uint32_t tiles = ((a.W + 7u) / 8u) * ((a.H + 7u) / 8u);
   const int frames = std::max(1, a.P.D - 1);
   int rc = launch_view(env, a, frames, tiles, scene->stack_depth);
+  if (rc != CG_OK) return rc;
+  CG_HIP(env, hipGetLastError());
+  return CG_OK;
+}
+
+int cg_view_job(cg_env* env, const cg_job* job) {
+  if (!env || !job || !job->scene || !job->history) return fail(env, CG_ERR_INVALID, "cg_view_job: null argument");
+  if (std::find(env->scenes.begin(), env->scenes.end(), job->scene) == env->scenes.end())
+    return fail(env, CG_ERR_INVALID, "job scene does not belong to this env");
+  const cg_conf& c = job->conf;
+  if (c.render_width == 0 || c.render_height == 0) return fail(env, CG_ERR_INVALID, "job render size must be positive");
+  ViewArgs a{};
+  a.S = job->scene->view;
+  a.P.D = static_cast<int>(std::min(std::max(c.trace_depth, 1u), 64u));
+  a.P.T = c.max_transport_sample_count;
+  a.P.E = c.max_emission_sample_count;
+  a.P.DIF = c.max_diffuse_sample_count;
+  a.P.ROUGH = c.max_rough_sample_count;
+  a.W = c.render_width;
+  a.H = c.render_height;
+  a.spp = c.sample_count;
+  a.iter = job->iter_count;
+  for (int k = 0; k < 3; ++k) {
+    a.cam_pos[k] = job->camera.position[k];
+    a.cam_dir[k] = job->camera.direction[k];
+  }
+  a.viewport[0] = job->camera.viewport[0];
+  a.viewport[1] = job->camera.viewport[1];
+  a.wl_cdf = static_cast<const float*>(env->wl_cdf.p);
+  a.wl_leaf = static_cast<const WlLeaf*>(env->wl_leaf.p);
+  a.wl_avg = env->wl_avg;
+  a.history = job->history;
+  a.rgba = reinterpret_cast<uint32_t*>(job->rgba8);
+  a.path_hash = job->path_hash;
+  a.ray_count = job->path_hash ? job->ray_count : nullptr;
+  a.stats = env->counting ? static_cast<unsigned long long*>(env->stats.p) : nullptr;
+  if (a.path_hash && !a.ray_count) return fail(env, CG_ERR_INVALID, "cg_view_job: path_hash needs ray_count");
+  const uint32_t tiles = ((a.W + 7u) / 8u) * ((a.H + 7u) / 8u);
+  int rc = launch_view(env, a, std::max(1, a.P.D - 1), tiles, job->scene->stack_depth);
   if (rc != CG_OK) return rc;
   CG_HIP(env, hipGetLastError());
   return CG_OK;

@@ -131,7 +131,7 @@ CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, const TStac
   rng_step(rng);
   out.rand_seed = rng;
   A.history[hidx] = out;
-  A.rgba[hidx] = xyz_to_rgba8(lum);
+  if (A.rgba) A.rgba[hidx] = xyz_to_rgba8(lum);
   if (A.path_hash) {
     A.path_hash[hidx] = hash;
     A.ray_count[hidx] = static_cast<uint32_t>(nrays + nshadow - rays_before);

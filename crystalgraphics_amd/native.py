@@ -209,6 +209,7 @@ def device() -> C.CDLL:
         _sig(d, "cg_device_free", I32, P, P)
         _sig(d, "cg_memcpy_h2d", I32, P, P, P, C.c_size_t)
         _sig(d, "cg_memcpy_d2h", I32, P, P, P, C.c_size_t)
+        _sig(d, "cg_view_job", I32, P, C.POINTER(Job))
         _sig(d, "cg_batch_create", I32, P, C.POINTER(BatchConf), C.POINTER(P))
         _sig(d, "cg_batch_destroy", I32, P)
         _sig(d, "cg_batch_footprint", I32, P, C.POINTER(C.c_uint64), C.POINTER(U32))

@@ -160,6 +160,10 @@ typedef struct cg_job {
   uint32_t* path_hash;         /* device, W*H, nullable (path records) */
   uint32_t* ray_count;         /* device, W*H, nullable */
 } cg_job;
+/* One job with the plain one-lane-per-pixel kernel of cg_view, on the job's
+ * own conf and buffers (cg_view renders the env's conf into the env's
+ * buffers).  The one-job-at-a-time form a batch is measured against. */
+int cg_view_job(cg_env* env, const cg_job* job);
 int cg_batch_create(cg_env* env, const cg_batch_conf* conf, cg_batch** out);
 int cg_batch_destroy(cg_batch* batch);
 /* Device bytes held, and the largest run of free slots (what fits now). */
