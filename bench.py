@@ -22,6 +22,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -135,6 +137,90 @@ def traversal_microbench(cg, ev, n_rays: int, n_tris: int, reps: int) -> dict:
     return out
 
 
+def batched_form(cg, seed: int, n_jobs: int, budget_gib: float, log) -> dict:
+    """The batched form: n_jobs render jobs (resolution log-uniform 64x64 to
+    1920x1080, spp in {1,4,16,64}, depth 2-12, camera on a radius-3 sphere
+    aimed at the origin) over 4 resident scenes built with the 16-sphere
+    generator at ~0.5M, 1M, 2M and 4M triangles.  Rendered one job at a time
+    with the plain kernel (cg_view_job), then together through the wavefront
+    batch engine (autotuned kernel shape and rounds per graph launch, slot
+    pool sized from the HBM budget); every job's history must be identical."""
+    import hashlib
+    from crystalgraphics_amd import autotune, workloads
+    t0 = time.perf_counter()
+    env = cg.DeviceEnv(cg.Conf.make(8, 8))
+    scenes, tris = [], []
+    for i, (name, sc) in enumerate(workloads.SCENE_SCALES.items()):
+        hs = cg.HostScene.generate("c1", seed=11 + i, params=[sc])
+        scenes.append(env.upload(hs.scene_data(share_blas=True, threads=16)))
+        tris.append(workloads.scene_triangles(sc))
+    jobs = workloads.batch_jobs(n_jobs, len(scenes), seed=seed)
+    depth = max(j.depth for j in jobs)
+    setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tune = autotune.tune_batch(env, scenes, jobs, depth, log=log)
+    tune_s = time.perf_counter() - t0
+    need = sum(((j.width + 7) // 8) * ((j.height + 7) // 8) * 64 for j in jobs)
+    tune.slot_capacity = autotune.capacity_for(int(budget_gib * (1 << 30)), depth, need)
+    rjs = [cg.RenderJob(env, scenes[j.scene], j.camera(), j.conf()) for j in jobs]
+    env.sync()
+    alone, t0 = [], time.perf_counter()
+    for rj in rjs:
+        a = time.perf_counter()
+        rj.view_alone()
+        env.sync()
+        alone.append(time.perf_counter() - a)
+    t_alone = time.perf_counter() - t0
+    ref = [hashlib.sha1(rj.read_history().tobytes()).hexdigest() for rj in rjs]
+    log(f"one job at a time: {t_alone:.2f} s")
+    b = cg.Batch(env, max_trace_depth=depth, **tune.kwargs())
+    footprint = b.footprint()[0]
+    b.stats(reset=True)
+    pending, done_at, by_id = list(range(len(rjs))), {}, {}
+    t0 = time.perf_counter()
+    launches = 0
+    while pending or len(done_at) < len(rjs):
+        still = []
+        for i in pending:
+            if b.submit(rjs[i]):
+                by_id[rjs[i].id] = i
+            else:
+                still.append(i)
+        pending = still
+        b.launch(1)
+        launches += 1
+        done, _ = b.poll()
+        now = time.perf_counter() - t0
+        for jid in done:
+            done_at[by_id.pop(jid)] = now
+    t_batch = time.perf_counter() - t0
+    st = b.stats()
+    b.close()
+    rays = st["rays"] + st["shadow_rays"]
+    same = sum(h == hashlib.sha1(rj.read_history().tobytes()).hexdigest() for h, rj in zip(ref, rjs))
+    for rj in rjs:
+        rj.free()
+    env.close()
+    lat = list(done_at.values())
+    seq = list(np.cumsum(alone))
+    pct = workloads.percentile
+    return {
+        "workload": f"{n_jobs} jobs, resolution log-uniform 64x64..1920x1080, spp {{1,4,16,64}}, depth 2-12, "
+                    f"over 4 resident scenes of {', '.join(f'{t/1e6:.2f}M' for t in tris)} tris",
+        "pixels": sum(j.pixels for j in jobs), "rays": rays, "setup_s": round(setup, 1),
+        "autotune": {"seconds": round(tune_s, 1), "chosen": tune.kwargs(), "trials": len(tune.trials)},
+        "one_at_a_time": {"seconds": round(t_alone, 3), "Mrays/s": round(rays / t_alone / 1e6, 1),
+                          "job_latency_ms": {"p50": round(pct(seq, 50) * 1e3, 1), "p99": round(pct(seq, 99) * 1e3, 1)},
+                          "job_render_ms": {"p50": round(pct(alone, 50) * 1e3, 1),
+                                            "p99": round(pct(alone, 99) * 1e3, 1)}},
+        "batched": {"seconds": round(t_batch, 3), "Mrays/s": round(rays / t_batch / 1e6, 1), "launches": launches,
+                    "job_latency_ms": {"p50": round(pct(lat, 50) * 1e3, 1), "p99": round(pct(lat, 99) * 1e3, 1)},
+                    "hbm_bytes": footprint},
+        "speedup": round(t_alone / t_batch, 3),
+        "identical_jobs": f"{same}/{len(rjs)}",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +231,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traversal", action="store_true", help="skip the 2^24-ray x 10M-triangle microbenchmark")
     ap.add_argument("--traversal-reps", type=int, default=3, help="timed launches per microbenchmark batch")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-form section")
+    ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--batch-budget-gib", type=float, default=16.0, help="HBM for the batch engine's slot pool")
     ap.add_argument("--probe-copy-gib", type=float, default=2.0,
                     help="size of the plain-copy HBM probe run before the frames (0: skip)")
     args = ap.parse_args()
@@ -237,6 +326,11 @@ def main() -> int:
             rl["traffic"] = round(tdata[k]["traffic_bytes"]) if tdata else None
             rl["traffic_source"] = os.path.relpath(tprof, ROOT) if tdata else None
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_form(cg, args.seed, args.batch_jobs, args.batch_budget_gib,
+                               lambda m: print(m, file=sys.stderr, flush=True))
+
     base = None
     if not args.no_cpu:
         base = cpu_baseline(scene, sd, conf, cam, args.cpu_seconds, min(16, os.cpu_count() or 1))
@@ -278,6 +372,7 @@ def main() -> int:
         "cpu_baseline": base,
         "hbm_probe": probe,
         "traversal_microbench": micro,
+        "batched_form": batched,
         "latency_ms": None,
     }
     print(json.dumps(out))

@@ -190,34 +190,35 @@ struct HitRec {
   uint32_t inst, slot;
 };
 
-// Traversal stack: the first kShortStack entries of each lane live in
+// Traversal stack: the first `short_n` entries of each lane live in
 // `ref`/`t` (LDS, `stride`-interleaved), deeper entries in `ovf` (device
 // memory, `ovf_stride`-interleaved; sized from the tree's stack bound).  An
 // entry is a node ref and the distance at which the ray enters that node.
-constexpr int kShortStack = 16;
+constexpr int kShortStack = 16;  // default LDS entries per lane
 struct TStack {
   uint32_t* ref;
   float* t;
   int stride;
+  int short_n;
   uint2* ovf;
   uint32_t ovf_stride;
   int ovf_cap;  // overflow entries per lane (the bound says none are needed beyond)
   CG_HD void push(int& sp, uint32_t r, float tt) const {
-    if (sp < kShortStack) {
+    if (sp < short_n) {
       ref[sp * stride] = r;
       t[sp * stride] = tt;
-    } else if (sp - kShortStack < ovf_cap) {
-      ovf[static_cast<size_t>(sp - kShortStack) * ovf_stride] = make_uint2(r, __float_as_uint(tt));
+    } else if (sp - short_n < ovf_cap) {
+      ovf[static_cast<size_t>(sp - short_n) * ovf_stride] = make_uint2(r, __float_as_uint(tt));
     }
     ++sp;
   }
   CG_HD void pop(int& sp, uint32_t& r, float& tt) const {
     --sp;
-    if (sp < kShortStack) {
+    if (sp < short_n) {
       r = ref[sp * stride];
       tt = t[sp * stride];
-    } else if (sp - kShortStack < ovf_cap) {
-      const uint2 e = ovf[static_cast<size_t>(sp - kShortStack) * ovf_stride];
+    } else if (sp - short_n < ovf_cap) {
+      const uint2 e = ovf[static_cast<size_t>(sp - short_n) * ovf_stride];
       r = e.x;
       tt = __uint_as_float(e.y);
     } else {

@@ -173,10 +173,13 @@ struct StackArgs {
   uint2* ovf;
   int ovf_cap;
 };
-__device__ __forceinline__ TStack lane_stack(uint32_t* s_ref, float* s_t, const StackArgs& sa) {
-  const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-  return TStack{s_ref + threadIdx.x, s_t + threadIdx.x, kBlock, sa.ovf ? sa.ovf + g : nullptr, gridDim.x * kBlock,
-                sa.ovf ? sa.ovf_cap : 0};
+// The stack of this lane: `short_n` entries in the block's LDS arrays, the
+// rest in the overflow area interleaved over the whole grid.
+template <int BLOCK>
+__device__ __forceinline__ TStack lane_stack(uint32_t* s_ref, float* s_t, const StackArgs& sa, int short_n) {
+  const uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
+  return TStack{s_ref + threadIdx.x, s_t + threadIdx.x, BLOCK, short_n, sa.ovf ? sa.ovf + g : nullptr,
+                gridDim.x * BLOCK, sa.ovf ? sa.ovf_cap : 0};
 }
 
 // Persistent traversal of queued rays with lane refill: a wave takes rays
@@ -248,7 +251,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_trace(DevScene S, const cg_ray* _
                                                   unsigned long long* __restrict__ stats, WorkQueue q, StackArgs sa) {
   __shared__ uint32_t s_ref[kShortStack * kBlock];
   __shared__ float s_t[kShortStack * kBlock];
-  const TStack stk = lane_stack(s_ref, s_t, sa);
+  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kShortStack);
   const TravScene scene = trav_scene(S);
   TravStats st{0u, 0u, 0u};
   uint64_t nq = 0;
@@ -299,7 +302,7 @@ template <int MF>
 __global__ __launch_bounds__(kBlock, 4) void k_view(ViewArgs A, WorkQueue q, StackArgs sa) {
   __shared__ uint32_t s_ref[kShortStack * kBlock];
   __shared__ float s_t[kShortStack * kBlock];
-  const TStack stk = lane_stack(s_ref, s_t, sa);
+  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kShortStack);
   const uint32_t tiles_x = (A.W + 7u) / 8u;
   TravStats st{0u, 0u, 0u};
   uint64_t nrays = 0, nshadow = 0;
@@ -370,11 +373,13 @@ __global__ __launch_bounds__(kBlock) void k_wf_admit(WfTables T, uint32_t j, uin
   wave_push(T.queue[0], T.counters + kWfQueueLen, push, slot);
 }
 
-// Trace the ray in flight of every slot on queue q.
-__global__ __launch_bounds__(kBlock, 5) void k_wf_trace(WfTables T, uint32_t q, StackArgs sa) {
-  __shared__ uint32_t s_ref[kShortStack * kBlock];
-  __shared__ float s_t[kShortStack * kBlock];
-  const TStack stk = lane_stack(s_ref, s_t, sa);
+// Trace the ray in flight of every slot on queue q.  BLOCK threads per
+// workgroup, SHORT LDS stack entries per lane (autotuned).
+template <int BLOCK, int SHORT>
+__global__ __launch_bounds__(BLOCK, 5) void k_wf_trace(WfTables T, uint32_t q, StackArgs sa) {
+  __shared__ uint32_t s_ref[SHORT * BLOCK];
+  __shared__ float s_t[SHORT * BLOCK];
+  const TStack stk = lane_stack<BLOCK>(s_ref, s_t, sa, SHORT);
   const uint32_t n = T.counters[kWfQueueLen + 32u * q];
   const WorkQueue wq{T.counters + kWfTraceWork, (n + 63u) / 64u};
   const uint32_t* queue = T.queue[q];
@@ -406,10 +411,11 @@ __global__ __launch_bounds__(kBlock, 5) void k_wf_trace(WfTables T, uint32_t q, 
 }
 
 // Advance every slot on queue q by one event; continuing slots go to q ^ 1.
-__global__ __launch_bounds__(kBlock, 4) void k_wf_shade(WfTables T, uint32_t q, StackArgs sa) {
-  __shared__ uint32_t s_ref[kShortStack * kBlock];
-  __shared__ float s_t[kShortStack * kBlock];
-  const TStack stk = lane_stack(s_ref, s_t, sa);
+template <int BLOCK, int SHORT>
+__global__ __launch_bounds__(BLOCK, 4) void k_wf_shade(WfTables T, uint32_t q, StackArgs sa) {
+  __shared__ uint32_t s_ref[SHORT * BLOCK];
+  __shared__ float s_t[SHORT * BLOCK];
+  const TStack stk = lane_stack<BLOCK>(s_ref, s_t, sa, SHORT);
   const uint32_t n = T.counters[kWfQueueLen + 32u * q];
   const WorkQueue wq{T.counters + kWfShadeWork, (n + 63u) / 64u};
   const uint32_t* queue = T.queue[q];
@@ -474,9 +480,9 @@ void release_scene(cg_scene* s) {
 // Resident grid of a kernel: blocks per CU at full occupancy x CUs, capped by
 // the number of work items.
 template <typename K>
-uint32_t resident_grid(K kernel, const cg_env* env, uint32_t items) {
+uint32_t resident_grid(K kernel, const cg_env* env, uint32_t items, int block = kBlock) {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1) per_cu = 1;
   const uint64_t g = static_cast<uint64_t>(per_cu) * static_cast<uint64_t>(env->cu_count > 0 ? env->cu_count : 1);
   return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(g, items)));
 }
@@ -930,6 +936,7 @@ struct cg_batch {
   WfTables T{};
   StackArgs sa{};
   uint32_t grid_trace = 0, grid_shade = 0;
+  void (*round)(cg_batch*, uint32_t) = nullptr;  // enqueue one round for the chosen kernel shape
   struct Job {
     bool active = false;
     uint32_t slot_base = 0, nslots = 0;
@@ -986,12 +993,35 @@ void slots_free(cg_batch* b, uint32_t first, uint32_t n) {
   }
 }
 
-void enqueue_round(cg_batch* b, uint32_t q) {
+template <int BLOCK, int SHORT>
+void round_t(cg_batch* b, uint32_t q) {
   hipStream_t st = b->env->stream;
-  hipLaunchKernelGGL(k_wf_trace, dim3(b->grid_trace), dim3(kBlock), 0, st, b->T, q, b->sa);
-  hipLaunchKernelGGL(k_wf_shade, dim3(b->grid_shade), dim3(kBlock), 0, st, b->T, q, b->sa);
+  hipLaunchKernelGGL((k_wf_trace<BLOCK, SHORT>), dim3(b->grid_trace), dim3(BLOCK), 0, st, b->T, q, b->sa);
+  hipLaunchKernelGGL((k_wf_shade<BLOCK, SHORT>), dim3(b->grid_shade), dim3(BLOCK), 0, st, b->T, q, b->sa);
   hipLaunchKernelGGL(k_wf_flip, dim3(1), dim3(64), 0, st, b->T, q);
 }
+template <int BLOCK, int SHORT>
+void shape_t(cg_batch* b, cg_env* env) {
+  b->round = round_t<BLOCK, SHORT>;
+  b->grid_trace = resident_grid(k_wf_trace<BLOCK, SHORT>, env, 1u << 30, BLOCK);
+  b->grid_shade = resident_grid(k_wf_shade<BLOCK, SHORT>, env, 1u << 30, BLOCK);
+}
+// Kernel shapes the autotuner chooses from: workgroup size x LDS stack entries.
+bool batch_shape(cg_batch* b, cg_env* env, uint32_t block, uint32_t short_n) {
+  switch (block * 100u + short_n) {
+    case 6408: shape_t<64, 8>(b, env); return true;
+    case 6416: shape_t<64, 16>(b, env); return true;
+    case 6424: shape_t<64, 24>(b, env); return true;
+    case 12808: shape_t<128, 8>(b, env); return true;
+    case 12816: shape_t<128, 16>(b, env); return true;
+    case 12824: shape_t<128, 24>(b, env); return true;
+    case 25608: shape_t<256, 8>(b, env); return true;
+    case 25616: shape_t<256, 16>(b, env); return true;
+    case 25624: shape_t<256, 24>(b, env); return true;
+    default: return false;
+  }
+}
+void enqueue_round(cg_batch* b, uint32_t q) { b->round(b, q); }
 
 int batch_build_graph(cg_batch* b) {
   cg_env* env = b->env;
@@ -1023,6 +1053,8 @@ int cg_batch_create(cg_env* env, const cg_batch_conf* conf, cg_batch** out) {
   if (c.max_jobs == 0 || c.max_jobs > 4096) return fail(env, CG_ERR_INVALID, "max_jobs must be in [1, 4096]");
   if (c.max_stack_depth == 0 || c.max_stack_depth > 4096)
     return fail(env, CG_ERR_INVALID, "max_stack_depth must be in [1, 4096]");
+  if (c.block_size == 0) c.block_size = kBlock;
+  if (c.short_stack == 0) c.short_stack = kShortStack;
   cg_batch* b = new cg_batch();
   b->env = env;
   b->conf = c;
@@ -1050,15 +1082,16 @@ int cg_batch_create(cg_env* env, const cg_batch_conf* conf, cg_batch** out) {
   alloc(b->scenes, kBatchMaxScenes * sizeof(DevScene));
   alloc(b->stats, kStatBytes);
   if (rc != CG_OK) return bail(rc);
-  b->grid_trace = resident_grid(k_wf_trace, env, 1u << 30);
-  b->grid_shade = resident_grid(k_wf_shade, env, 1u << 30);
+  if (!batch_shape(b, env, c.block_size, c.short_stack))
+    return bail(fail(env, CG_ERR_INVALID, "unsupported kernel shape: block %u, short stack %u", c.block_size,
+                     c.short_stack));
   // The traversal-stack overflow area is the batch's own (its address is
   // baked into the graph), sized for the larger of the two resident grids.
   b->sa.ovf = nullptr;
   b->sa.ovf_cap = 0;
-  if (c.max_stack_depth > static_cast<uint32_t>(kShortStack)) {
-    b->sa.ovf_cap = static_cast<int>(c.max_stack_depth) - kShortStack;
-    alloc(b->ovf, static_cast<size_t>(std::max(b->grid_trace, b->grid_shade)) * kBlock *
+  if (c.max_stack_depth > c.short_stack) {
+    b->sa.ovf_cap = static_cast<int>(c.max_stack_depth - c.short_stack);
+    alloc(b->ovf, static_cast<size_t>(std::max(b->grid_trace, b->grid_shade)) * c.block_size *
                       static_cast<size_t>(b->sa.ovf_cap) * sizeof(uint2));
     if (rc != CG_OK) return bail(rc);
     b->sa.ovf = static_cast<uint2*>(b->ovf.p);

@@ -101,7 +101,8 @@ class SceneDescC(C.Structure):
 class BatchConf(C.Structure):
     """cg_batch_conf"""
     _fields_ = [("slot_capacity", C.c_uint32), ("max_trace_depth", C.c_uint32), ("rounds_per_launch", C.c_uint32),
-                ("use_graph", C.c_uint32), ("max_jobs", C.c_uint32), ("max_stack_depth", C.c_uint32)]
+                ("use_graph", C.c_uint32), ("max_jobs", C.c_uint32), ("max_stack_depth", C.c_uint32),
+                ("block_size", C.c_uint32), ("short_stack", C.c_uint32)]
 
 
 class Job(C.Structure):

@@ -148,6 +148,8 @@ typedef struct cg_batch_conf {
   uint32_t use_graph;          /* 1: one launch = one replay of a captured hipGraph */
   uint32_t max_jobs;           /* job table entries (1-4096) */
   uint32_t max_stack_depth;    /* deepest traversal stack of an accepted scene */
+  uint32_t block_size;         /* workgroup size of the round kernels: 64, 128 or 256 (0: 256) */
+  uint32_t short_stack;        /* LDS traversal-stack entries per lane: 8, 16 or 24 (0: 16) */
 } cg_batch_conf;
 typedef struct cg_job {
   cg_scene* scene;

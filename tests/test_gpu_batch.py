@@ -67,11 +67,12 @@ def alone(cg, scenes):
     return [_alone(cg, scenes, j) for j in _jobs()]
 
 
-def _run_batch(cg, scenes, jobs, use_graph=True, staggered=False, slot_capacity=1 << 16, rounds=8):
+def _run_batch(cg, scenes, jobs, use_graph=True, staggered=False, slot_capacity=1 << 16, rounds=8,
+               block_size=256, short_stack=16):
     env = cg.DeviceEnv(cg.Conf.make(8, 8))
     ds = {n: env.upload(sd) for n, (s, sd) in scenes.items()}
     b = cg.Batch(env, slot_capacity=slot_capacity, max_trace_depth=8, rounds_per_launch=rounds,
-                 use_graph=use_graph)
+                 use_graph=use_graph, block_size=block_size, short_stack=short_stack)
     rjs = []
     for i, (name, w, h, spp, depth, k) in enumerate(jobs):
         s, _ = scenes[name]
@@ -106,6 +107,15 @@ def test_batch_equals_view(cg, scenes, alone, use_graph):
     # every query the path records count was traced by the batch
     total = sum(int(g[2][1].sum()) for g in got)
     assert st["rays"] + st["shadow_rays"] == total
+
+
+@pytest.mark.parametrize("block_size,short_stack", [(b, k) for b in (64, 128, 256) for k in (8, 16, 24)])
+def test_batch_kernel_shapes(cg, scenes, alone, block_size, short_stack):
+    """Every kernel shape the autotuner can pick gives the same results (the
+    short stack spills to the overflow area at a different depth)."""
+    got, _ = _run_batch(cg, scenes, _jobs()[:3], block_size=block_size, short_stack=short_stack)
+    for i, (g, r) in enumerate(zip(got, alone[:3])):
+        _check_equal(g, r, f"job {i}")
 
 
 def test_batch_staggered_admission(cg, scenes, alone):
