@@ -20,7 +20,7 @@ from . import native
 from .native import (ALIAS, BLAS_NODE, EMISSIVE, HISTORY, HIT, INDEX, INSTANCE, MATERIAL, MISS_ID, MISS_T,
                      PRIMITIVE, RAY, TLAS_NODE, VERTEX, Camera, Conf, NativeError)
 
-__all__ = ["HostScene", "SceneData", "DeviceEnv", "DeviceScene", "Camera", "Conf", "NativeError", "Batch", "RenderJob",
+__all__ = ["HostScene", "SceneData", "DeviceEnv", "DeviceScene", "Camera", "Conf", "NativeError", "Batch", "RenderJob", "Server",
            "EnvInit", "EnvTerm", "EnvPoll", "EnvPresent", "EnvStatus", "LoadScene", "View", "ViewToBuffer",
            "MISS_ID", "MISS_T", "RAY", "HIT", "HISTORY", "VERTEX", "MATERIAL", "PRIMITIVE", "TLAS_NODE",
            "INSTANCE", "BLAS_NODE", "INDEX", "EMISSIVE", "ALIAS"]
@@ -424,6 +424,68 @@ class Batch:
     def close(self) -> None:
         if self._h:
             native.device().cg_batch_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Server:
+    """The serving form (crystal::graphics::Server, include/CrystalGraphics/
+    serve.h): scenes resident in HBM, requests admitted within an HBM budget
+    and batched as they arrive; each result equals a plain View's."""
+
+    MAX_PIXELS = 1920 * 1080
+
+    def __init__(self, scenes: list[HostScene], budget_bytes: int = 16 << 30, output_fraction: float = 0.25,
+                 max_trace_depth: int = 8, rounds_per_launch: int = 16, block_size: int = 0, short_stack: int = 0):
+        arr = (C.c_void_p * len(scenes))(*[s.handle for s in scenes])
+        h = C.c_void_p()
+        rc = native.host().cgh_server_create(arr, len(scenes), int(budget_bytes), float(output_fraction),
+                                             max_trace_depth, rounds_per_launch, block_size, short_stack, C.byref(h))
+        if rc != 0:
+            raise NativeError(rc, native.host().cgh_serve_last_error().decode())
+        self._h = h.value
+        self._buf = np.zeros(self.MAX_PIXELS, HISTORY)
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise NativeError(rc, native.host().cgh_serve_last_error().decode())
+
+    def submit(self, scene: int, camera: Camera, conf: Conf) -> int:
+        rid = C.c_uint64()
+        self._check(native.host().cgh_server_submit(self._h, scene, C.byref(camera), C.byref(conf), C.byref(rid)))
+        return rid.value
+
+    def _result(self, info) -> dict:
+        n = info.width * info.height
+        return {"id": info.id, "history": self._buf[:n].copy().reshape(info.height, info.width),
+                "queued_s": info.queued_s, "latency_s": info.latency_s}
+
+    def next(self, timeout: float = 1.0) -> dict | None:
+        """The next finished request, or None if none finished in time."""
+        info = native.ResultInfo()
+        self._check(native.host().cgh_server_next(self._h, timeout, C.byref(info), self._buf.ctypes.data,
+                                                  self._buf.shape[0]))
+        return self._result(info) if info.id else None
+
+    def render_alone(self, scene: int, camera: Camera, conf: Conf) -> dict:
+        info = native.ResultInfo()
+        self._check(native.host().cgh_server_render_alone(self._h, scene, C.byref(camera), C.byref(conf),
+                                                          C.byref(info), self._buf.ctypes.data, self._buf.shape[0]))
+        return self._result(info)
+
+    def stats(self) -> dict:
+        s = native.ServeStats()
+        self._check(native.host().cgh_server_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in native.ServeStats._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            native.host().cgh_server_free(self._h)
             self._h = None
 
     def __enter__(self):

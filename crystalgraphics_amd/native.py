@@ -112,6 +112,18 @@ class Job(C.Structure):
                 ("ray_count", C.c_void_p)]
 
 
+class ResultInfo(C.Structure):
+    """cgh_result_info"""
+    _fields_ = [("id", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32), ("queued_s", C.c_double),
+                ("latency_s", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    """cgh_serve_stats"""
+    _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64), ("rays", C.c_uint64),
+                ("gpu_busy_s", C.c_double)]
+
+
 class Stats(C.Structure):
     _fields_ = [("rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("nodes", C.c_uint64),
                 ("tris", C.c_uint64), ("insts", C.c_uint64)]
@@ -246,6 +258,15 @@ def host() -> C.CDLL:
         _sig(h, "cgh_view", I32, P, C.POINTER(Camera))
         _sig(h, "cgh_view_to_buffer", I32, P, C.POINTER(Camera), P, C.c_size_t)
         _sig(h, "cgh_load_scene", I32, C.c_char_p, C.POINTER(P))
+        _sig(h, "cgh_serve_last_error", C.c_char_p)
+        _sig(h, "cgh_server_create", I32, C.POINTER(P), U32, C.c_uint64, C.c_double, U32, U32, U32, U32,
+             C.POINTER(P))
+        _sig(h, "cgh_server_free", None, P)
+        _sig(h, "cgh_server_submit", I32, P, U32, C.POINTER(Camera), C.POINTER(Conf), C.POINTER(C.c_uint64))
+        _sig(h, "cgh_server_next", I32, P, C.c_double, C.POINTER(ResultInfo), P, C.c_size_t)
+        _sig(h, "cgh_server_render_alone", I32, P, U32, C.POINTER(Camera), C.POINTER(Conf), C.POINTER(ResultInfo),
+             P, C.c_size_t)
+        _sig(h, "cgh_server_stats", I32, P, C.POINTER(ServeStats))
         _host = h
     return _host
 

@@ -6,5 +6,6 @@
 #include "environment.h"
 #include "error.h"
 #include "scene.h"
+#include "serve.h"
 
 #endif
