@@ -221,6 +221,66 @@ def batched_form(cg, seed: int, n_jobs: int, budget_gib: float, log) -> dict:
     }
 
 
+def serving_form(cg, seed: int, rate: float, seconds: float, n_scenes: int, scene_scale: float, budget_gib: float,
+                 check: int, log) -> dict:
+    """The serving form: n_scenes resident scenes (16-sphere generator at
+    `scene_scale`), Poisson arrivals at `rate` per second for `seconds`; each
+    request: a scene, a camera on the orbit sphere, a resolution from
+    {256x256, 640x360, 1280x720, 1920x1080} weighted 4:3:2:1, spp in
+    {1, 4, 16}, depth 8.  Requests are admitted within the HBM budget and
+    batched as they arrive; latency is arrival to result.  A sample of
+    `check` requests is rendered again alone with the plain kernel and
+    compared (relative MSE of the XYZ image)."""
+    from crystalgraphics_amd import workloads
+    t0 = time.perf_counter()
+    hosts = [cg.HostScene.generate("c1", seed=21 + i, params=[scene_scale]) for i in range(n_scenes)]
+    tris = workloads.scene_triangles(scene_scale) * n_scenes
+    reqs = workloads.serve_requests(rate, seconds, n_scenes, seed=seed)
+    srv = cg.Server(hosts, budget_bytes=int(budget_gib * (1 << 30)), max_trace_depth=8)
+    setup = time.perf_counter() - t0
+    log(f"serving: {len(reqs)} requests over {seconds:.0f} s at {rate:.0f}/s, setup {setup:.1f} s")
+    ids, lat, queued, kept = {}, {}, {}, {}
+    check_set = set(list(range(len(reqs)))[:: max(1, len(reqs) // max(1, check))][:check])
+    i, t_start = 0, time.perf_counter()
+    t_last = t_start
+    while len(lat) < len(reqs):
+        now = time.perf_counter() - t_start
+        while i < len(reqs) and reqs[i][0] <= now:
+            spec = reqs[i][1]
+            ids[srv.submit(spec.scene, spec.camera(), spec.conf())] = i
+            i += 1
+        wait = (reqs[i][0] - now) if i < len(reqs) else 0.25
+        r = srv.next(max(0.0, min(wait, 0.05)))
+        if r is not None:
+            k = ids[r["id"]]
+            lat[k], queued[k] = r["latency_s"], r["queued_s"]
+            if k in check_set:
+                kept[k] = r["history"]["xyz"].astype(np.float64)
+            t_last = time.perf_counter()
+    span = t_last - t_start
+    st = srv.stats()
+    rmse = []  # each checked request as served in the mix vs rendered alone with the plain kernel
+    for k in sorted(kept):
+        spec = reqs[k][1]
+        a = srv.render_alone(spec.scene, spec.camera(), spec.conf())["history"]["xyz"].astype(np.float64)
+        rmse.append(float(((kept[k] - a) ** 2).sum() / max((a ** 2).sum(), 1e-30)))
+    srv.close()
+    pct = workloads.percentile
+    lv = list(lat.values())
+    return {
+        "workload": f"{n_scenes} resident scenes, {tris / 1e6:.1f}M tris total; Poisson {rate:.0f} req/s for "
+                    f"{seconds:.0f} s; 256x256/640x360/1280x720/1920x1080 weighted 4:3:2:1, spp {{1,4,16}}, depth 8",
+        "requests": len(reqs), "setup_s": round(setup, 1), "budget_gib": budget_gib,
+        "latency_ms": {"p50": round(pct(lv, 50) * 1e3, 1), "p99": round(pct(lv, 99) * 1e3, 1),
+                       "max": round(max(lv) * 1e3, 1)},
+        "queued_ms": {"p50": round(pct(list(queued.values()), 50) * 1e3, 1),
+                      "p99": round(pct(list(queued.values()), 99) * 1e3, 1)},
+        "rays": st["rays"], "Mrays/s": round(st["rays"] / span / 1e6, 1), "span_s": round(span, 2),
+        "gpu_busy_frac": round(st["gpu_busy_s"] / span, 3),
+        "rmse_vs_plain": {"checked": len(rmse), "max": max(rmse) if rmse else None},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,6 +292,13 @@ def main() -> int:
     ap.add_argument("--no-traversal", action="store_true", help="skip the 2^24-ray x 10M-triangle microbenchmark")
     ap.add_argument("--traversal-reps", type=int, default=3, help="timed launches per microbenchmark batch")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form section")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-form section")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="requests per second (Poisson)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-scenes", type=int, default=8)
+    ap.add_argument("--serve-scene-scale", type=float, default=1.5390625,
+                    help="sphere subdivision factor of each scene (1.539: ~4.9M tris per scene)")
+    ap.add_argument("--serve-budget-gib", type=float, default=24.0)
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--batch-budget-gib", type=float, default=16.0, help="HBM for the batch engine's slot pool")
     ap.add_argument("--probe-copy-gib", type=float, default=2.0,
@@ -331,6 +398,12 @@ def main() -> int:
         batched = batched_form(cg, args.seed, args.batch_jobs, args.batch_budget_gib,
                                lambda m: print(m, file=sys.stderr, flush=True))
 
+    serving = None
+    if not args.no_serve:
+        serving = serving_form(cg, args.seed, args.serve_rate, args.serve_seconds, args.serve_scenes,
+                               args.serve_scene_scale, args.serve_budget_gib, 4,
+                               lambda m: print(m, file=sys.stderr, flush=True))
+
     base = None
     if not args.no_cpu:
         base = cpu_baseline(scene, sd, conf, cam, args.cpu_seconds, min(16, os.cpu_count() or 1))
@@ -373,7 +446,8 @@ def main() -> int:
         "hbm_probe": probe,
         "traversal_microbench": micro,
         "batched_form": batched,
-        "latency_ms": None,
+        "serving_form": serving,
+        "latency_ms": None if serving is None else serving["latency_ms"],
     }
     print(json.dumps(out))
     env.close()
