@@ -20,7 +20,7 @@ from . import native
 from .native import (ALIAS, BLAS_NODE, EMISSIVE, HISTORY, HIT, INDEX, INSTANCE, MATERIAL, MISS_ID, MISS_T,
                      PRIMITIVE, RAY, TLAS_NODE, VERTEX, Camera, Conf, NativeError)
 
-__all__ = ["HostScene", "SceneData", "DeviceEnv", "DeviceScene", "Camera", "Conf", "NativeError", "Batch", "RenderJob", "Server",
+__all__ = ["HostScene", "SceneData", "DeviceEnv", "DeviceScene", "Camera", "Conf", "NativeError", "Batch", "RenderJob", "Server", "ViewBatch",
            "EnvInit", "EnvTerm", "EnvPoll", "EnvPresent", "EnvStatus", "LoadScene", "View", "ViewToBuffer",
            "MISS_ID", "MISS_T", "RAY", "HIT", "HISTORY", "VERTEX", "MATERIAL", "PRIMITIVE", "TLAS_NODE",
            "INSTANCE", "BLAS_NODE", "INDEX", "EMISSIVE", "ALIAS"]
@@ -372,6 +372,32 @@ class RenderJob:
         for b in (self.history, self.rgba, self.path_hash, self.ray_count):
             if b:
                 b.free()
+
+
+class ViewBatch:
+    """Many jobs in one persistent launch of the plain per-pixel kernel
+    (cg_view_batch_launch), on a stream of its own; results equal
+    RenderJob.view_alone's."""
+
+    def __init__(self, env: DeviceEnv, jobs: list[RenderJob]):
+        self.env, self.jobs = env, list(jobs)
+        arr = (native.Job * len(self.jobs))(*[j.c() for j in self.jobs])
+        h = C.c_void_p()
+        env._check(native.device().cg_view_batch_launch(env.handle, arr, len(self.jobs), C.byref(h)))
+        self._h = h.value
+
+    def done(self) -> bool:
+        d = C.c_int32()
+        self.env._check(native.device().cg_view_batch_query(self._h, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self) -> None:
+        self.env._check(native.device().cg_view_batch_wait(self._h))
+
+    def release(self) -> None:
+        if self._h:
+            self.env._check(native.device().cg_view_batch_release(self._h))
+            self._h = None
 
 
 class Batch:

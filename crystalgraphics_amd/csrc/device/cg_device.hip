@@ -39,6 +39,16 @@ struct cg_scene {
   uint64_t bytes = 0;
 };
 
+struct cg_vbatch {
+  cg_env* env = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  DevBuf jobs, first, queue, ovf;
+  std::vector<ViewArgs> host_jobs;  // sources of the in-flight copies: kept until reuse
+  std::vector<uint32_t> host_first;
+  bool in_flight = false;
+};
+
 struct cg_env {
   cg_conf conf{};
   int device = 0;
@@ -52,6 +62,8 @@ struct cg_env {
   bool path_records = false;
   int accel = CG_ACCEL_REBUILT;
   std::vector<cg_scene*> scenes;
+  std::vector<cg_vbatch*> vbatch_pool;  // finished view batches, for reuse
+  std::vector<cg_vbatch*> vbatch_all;
 };
 
 namespace {
@@ -316,6 +328,43 @@ __global__ __launch_bounds__(kBlock, 4) void k_view(ViewArgs A, WorkQueue q, Sta
     if (px < A.W && py < A.H) render_pixel<MF>(A, px, py, stk, st, nrays, nshadow);
   }
   if (A.stats) flush_stats(A.stats, nrays, nshadow, st.nodes, st.tris, st.insts);
+}
+
+// Many render jobs in one persistent launch of the same per-pixel kernel:
+// the work items are the 8x8 tiles of every job, job j owning items
+// [first[j], first[j+1]).  A wave finding a job's heavy tiles no longer leaves
+// the rest of the GPU idle: other jobs' tiles fill it.
+template <int MF>
+__global__ __launch_bounds__(kBlock, 4) void k_view_batch(const ViewArgs* __restrict__ jobs,
+                                                          const uint32_t* __restrict__ first, uint32_t njobs,
+                                                          WorkQueue q, StackArgs sa,
+                                                          unsigned long long* __restrict__ stats) {
+  __shared__ uint32_t s_ref[kShortStack * kBlock];
+  __shared__ float s_t[kShortStack * kBlock];
+  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kShortStack);
+  TravStats st{0u, 0u, 0u};
+  uint64_t nrays = 0, nshadow = 0;
+  uint32_t probe = 0;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (;;) {
+    const uint32_t item = grab_item(q, probe);
+    if (item == kNoItem) break;
+    uint32_t lo = 0, hi = njobs;  // the job with first[lo] <= item < first[lo + 1]
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (first[mid] <= item)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const ViewArgs& A = jobs[lo];
+    const uint32_t tile = item - first[lo];
+    const uint32_t tiles_x = (A.W + 7u) / 8u;
+    const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u);
+    const uint32_t py = (tile / tiles_x) * 8u + (lane >> 3);
+    if (px < A.W && py < A.H) render_pixel<MF>(A, px, py, stk, st, nrays, nshadow);
+  }
+  if (stats) flush_stats(stats, nrays, nshadow, st.nodes, st.tris, st.insts);
 }
 
 // Plain streaming copy: 16 B per lane per iteration, grid-stride.
@@ -612,6 +661,15 @@ int cg_env_destroy(cg_env* env) {
   dev_free(env->wl_leaf);
   dev_free(env->queue);
   dev_free(env->ovf);
+  for (cg_vbatch* v : env->vbatch_all) {
+    if (v->stream) (void)hipStreamSynchronize(v->stream);
+    for (DevBuf* d : {&v->jobs, &v->first, &v->queue, &v->ovf}) dev_free(*d);
+    if (v->done) (void)hipEventDestroy(v->done);
+    if (v->stream) (void)hipStreamDestroy(v->stream);
+    delete v;
+  }
+  env->vbatch_all.clear();
+  env->vbatch_pool.clear();
   if (env->stream) (void)hipStreamDestroy(env->stream);
   delete env;
   return CG_OK;
@@ -829,6 +887,164 @@ int cg_view_job(cg_env* env, const cg_job* job) {
   int rc = launch_view(env, a, std::max(1, a.P.D - 1), tiles, job->scene->stack_depth);
   if (rc != CG_OK) return rc;
   CG_HIP(env, hipGetLastError());
+  return CG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+int grow(cg_env* env, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return CG_OK;
+  dev_free(b);
+  return dev_alloc(env, b, bytes);
+}
+template <int MF>
+void launch_view_batch_mf(cg_vbatch* v, uint32_t grid, uint32_t njobs, const WorkQueue& q, const StackArgs& sa,
+                          unsigned long long* stats) {
+  hipLaunchKernelGGL((k_view_batch<MF>), dim3(grid), dim3(kBlock), 0, v->stream,
+                     static_cast<const ViewArgs*>(v->jobs.p), static_cast<const uint32_t*>(v->first.p), njobs, q, sa,
+                     stats);
+}
+template <int MF>
+uint32_t view_batch_grid(cg_env* env, uint32_t items) {
+  return resident_grid(k_view_batch<MF>, env, (items + 3u) / 4u);
+}
+}  // namespace
+
+extern "C" {
+
+int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch** out) {
+  if (!env || !out || (n && !jobs)) return fail(env, CG_ERR_INVALID, "cg_view_batch_launch: null argument");
+  *out = nullptr;
+  if (n == 0) return fail(env, CG_ERR_INVALID, "cg_view_batch_launch: no jobs");
+  std::vector<ViewArgs> va(n);
+  std::vector<uint32_t> first(n + 1, 0u);
+  uint32_t depth = 1, frames = 1;
+  for (uint32_t i = 0; i < n; ++i) {
+    const cg_job* job = jobs + i;
+    if (!job->scene || !job->history) return fail(env, CG_ERR_INVALID, "job %u: null scene or history", i);
+    if (std::find(env->scenes.begin(), env->scenes.end(), job->scene) == env->scenes.end())
+      return fail(env, CG_ERR_INVALID, "job %u: scene does not belong to this env", i);
+    const cg_conf& c = job->conf;
+    if (c.render_width == 0 || c.render_height == 0) return fail(env, CG_ERR_INVALID, "job %u: empty render size", i);
+    if (job->path_hash && !job->ray_count) return fail(env, CG_ERR_INVALID, "job %u: path_hash needs ray_count", i);
+    ViewArgs& a = va[i];
+    a = ViewArgs{};
+    a.S = job->scene->view;
+    a.P.D = static_cast<int>(std::min(std::max(c.trace_depth, 1u), 64u));
+    a.P.T = c.max_transport_sample_count;
+    a.P.E = c.max_emission_sample_count;
+    a.P.DIF = c.max_diffuse_sample_count;
+    a.P.ROUGH = c.max_rough_sample_count;
+    a.W = c.render_width;
+    a.H = c.render_height;
+    a.spp = c.sample_count;
+    a.iter = job->iter_count;
+    for (int k = 0; k < 3; ++k) {
+      a.cam_pos[k] = job->camera.position[k];
+      a.cam_dir[k] = job->camera.direction[k];
+    }
+    a.viewport[0] = job->camera.viewport[0];
+    a.viewport[1] = job->camera.viewport[1];
+    a.wl_cdf = static_cast<const float*>(env->wl_cdf.p);
+    a.wl_leaf = static_cast<const WlLeaf*>(env->wl_leaf.p);
+    a.wl_avg = env->wl_avg;
+    a.history = job->history;
+    a.rgba = reinterpret_cast<uint32_t*>(job->rgba8);
+    a.path_hash = job->path_hash;
+    a.ray_count = job->path_hash ? job->ray_count : nullptr;
+    a.stats = nullptr;
+    const uint64_t tiles = static_cast<uint64_t>((c.render_width + 7u) / 8u) * ((c.render_height + 7u) / 8u);
+    if (first[i] + tiles > 0xFFFFFF00ull) return fail(env, CG_ERR_INVALID, "batch too large");
+    first[i + 1] = first[i] + static_cast<uint32_t>(tiles);
+    depth = std::max(depth, job->scene->stack_depth);
+    frames = std::max(frames, static_cast<uint32_t>(std::max(1, a.P.D - 1)));
+  }
+  cg_vbatch* v = nullptr;
+  if (!env->vbatch_pool.empty()) {
+    v = env->vbatch_pool.back();
+    env->vbatch_pool.pop_back();
+  } else {
+    v = new cg_vbatch();
+    v->env = env;
+    env->vbatch_all.push_back(v);
+    if (hipStreamCreateWithFlags(&v->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&v->done, hipEventDisableTiming) != hipSuccess) {
+      env->vbatch_pool.push_back(v);
+      return fail(env, CG_ERR_HIP, "cg_view_batch_launch: stream or event creation failed");
+    }
+  }
+  auto give_back = [&](int rc) {
+    env->vbatch_pool.push_back(v);
+    return rc;
+  };
+  const uint32_t items = first[n];
+  const uint32_t grid = frames <= 4    ? view_batch_grid<4>(env, items)
+                        : frames <= 8  ? view_batch_grid<8>(env, items)
+                        : frames <= 16 ? view_batch_grid<16>(env, items)
+                                       : view_batch_grid<64>(env, items);
+  int rc = CG_OK;
+  if ((rc = grow(env, v->jobs, n * sizeof(ViewArgs))) != CG_OK) return give_back(rc);
+  if ((rc = grow(env, v->first, (n + 1) * sizeof(uint32_t))) != CG_OK) return give_back(rc);
+  if ((rc = grow(env, v->queue, kXcds * kCounterStride * sizeof(uint32_t))) != CG_OK) return give_back(rc);
+  StackArgs sa{nullptr, 0};
+  if (depth > static_cast<uint32_t>(kShortStack)) {
+    sa.ovf_cap = static_cast<int>(depth) - kShortStack;
+    if ((rc = grow(env, v->ovf, static_cast<size_t>(grid) * kBlock * sa.ovf_cap * sizeof(uint2))) != CG_OK)
+      return give_back(rc);
+    sa.ovf = static_cast<uint2*>(v->ovf.p);
+  }
+  v->host_jobs = std::move(va);
+  v->host_first = std::move(first);
+  if (hipMemcpyAsync(v->jobs.p, v->host_jobs.data(), n * sizeof(ViewArgs), hipMemcpyHostToDevice, v->stream) !=
+          hipSuccess ||
+      hipMemcpyAsync(v->first.p, v->host_first.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                     v->stream) != hipSuccess)
+    return give_back(fail(env, CG_ERR_HIP, "cg_view_batch_launch: job table copy failed"));
+  uint32_t* ctr = static_cast<uint32_t*>(v->queue.p);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kXcds * kCounterStride), 0, v->stream, ctr, kXcds * kCounterStride);
+  const WorkQueue q{ctr, items};
+  unsigned long long* stats = env->counting ? static_cast<unsigned long long*>(env->stats.p) : nullptr;
+  if (frames <= 4)
+    launch_view_batch_mf<4>(v, grid, n, q, sa, stats);
+  else if (frames <= 8)
+    launch_view_batch_mf<8>(v, grid, n, q, sa, stats);
+  else if (frames <= 16)
+    launch_view_batch_mf<16>(v, grid, n, q, sa, stats);
+  else
+    launch_view_batch_mf<64>(v, grid, n, q, sa, stats);
+  if (hipGetLastError() != hipSuccess || hipEventRecord(v->done, v->stream) != hipSuccess)
+    return give_back(fail(env, CG_ERR_HIP, "cg_view_batch_launch: launch failed"));
+  v->in_flight = true;
+  *out = v;
+  return CG_OK;
+}
+
+int cg_view_batch_query(cg_vbatch* v, int* done) {
+  if (!v || !done) return fail(nullptr, CG_ERR_INVALID, "cg_view_batch_query: null argument");
+  const hipError_t e = hipEventQuery(v->done);
+  if (e == hipSuccess) {
+    *done = 1;
+    return CG_OK;
+  }
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return CG_OK;
+  }
+  return fail(v->env, CG_ERR_HIP, "cg_view_batch_query: %s", hipGetErrorString(e));
+}
+
+int cg_view_batch_wait(cg_vbatch* v) {
+  if (!v) return fail(nullptr, CG_ERR_INVALID, "cg_view_batch_wait: null batch");
+  CG_HIP(v->env, hipEventSynchronize(v->done));
+  return CG_OK;
+}
+
+int cg_view_batch_release(cg_vbatch* v) {
+  if (!v) return fail(nullptr, CG_ERR_INVALID, "cg_view_batch_release: null batch");
+  CG_HIP(v->env, hipEventSynchronize(v->done));
+  v->in_flight = false;
+  v->env->vbatch_pool.push_back(v);
   return CG_OK;
 }
 

@@ -223,6 +223,10 @@ def device() -> C.CDLL:
         _sig(d, "cg_memcpy_h2d", I32, P, P, P, C.c_size_t)
         _sig(d, "cg_memcpy_d2h", I32, P, P, P, C.c_size_t)
         _sig(d, "cg_view_job", I32, P, C.POINTER(Job))
+        _sig(d, "cg_view_batch_launch", I32, P, C.POINTER(Job), U32, C.POINTER(P))
+        _sig(d, "cg_view_batch_query", I32, P, C.POINTER(I32))
+        _sig(d, "cg_view_batch_wait", I32, P)
+        _sig(d, "cg_view_batch_release", I32, P)
         _sig(d, "cg_batch_create", I32, P, C.POINTER(BatchConf), C.POINTER(P))
         _sig(d, "cg_batch_destroy", I32, P)
         _sig(d, "cg_batch_footprint", I32, P, C.POINTER(C.c_uint64), C.POINTER(U32))

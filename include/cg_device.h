@@ -166,6 +166,18 @@ typedef struct cg_job {
  * own conf and buffers (cg_view renders the env's conf into the env's
  * buffers).  The one-job-at-a-time form a batch is measured against. */
 int cg_view_job(cg_env* env, const cg_job* job);
+/* Many jobs in one persistent launch of the plain per-pixel kernel: the work
+ * items are the 8x8 tiles of all n jobs (in the order given), so the tiles of
+ * other jobs fill the GPU while one job's heavy tiles finish.  Each job's
+ * result equals cg_view_job's, bit for bit.  The launch runs on a stream of
+ * its own (taken from a pool; default priority), so successive launches
+ * overlap; query / wait on the returned handle, then release it.  The job
+ * table is copied at launch; buffers must stay valid until it finishes. */
+typedef struct cg_vbatch cg_vbatch;
+int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch** out);
+int cg_view_batch_query(cg_vbatch* batch, int* done);
+int cg_view_batch_wait(cg_vbatch* batch);
+int cg_view_batch_release(cg_vbatch* batch);  /* waits, then returns the stream to the pool */
 int cg_batch_create(cg_env* env, const cg_batch_conf* conf, cg_batch** out);
 int cg_batch_destroy(cg_batch* batch);
 /* Device bytes held, and the largest run of free slots (what fits now). */

@@ -171,3 +171,26 @@ def test_batch_progressive_history(cg, scenes):
         j.free()
     b.close()
     env.close()
+
+
+def test_view_batch_equals_view(cg, scenes, alone):
+    """All jobs in one persistent launch of the per-pixel kernel (and a second
+    launch overlapping the first on another stream) give cg_view's results."""
+    env = cg.DeviceEnv(cg.Conf.make(8, 8))
+    ds = {n: env.upload(sd) for n, (s, sd) in scenes.items()}
+    mk = lambda: [cg.RenderJob(env, ds[name], _camera(cg, scenes[name][0].describe(), k), cg.Conf.make(w, h, spp, d),
+                               path_records=True) for (name, w, h, spp, d, k) in _jobs()]
+    a, b = mk(), mk()
+    va = cg.ViewBatch(env, a)
+    vb = cg.ViewBatch(env, b[::-1])  # another order, in flight at the same time
+    va.wait()
+    vb.wait()
+    assert va.done() and vb.done()
+    va.release()
+    vb.release()
+    for jobs in (a, b):
+        for i, (rj, r) in enumerate(zip(jobs, alone)):
+            _check_equal((rj.read_history(), rj.read_rgba8(), rj.read_path_records()), r, f"job {i}")
+    for rj in a + b:
+        rj.free()
+    env.close()
