@@ -125,7 +125,13 @@ struct Server::Impl {
 
   // server-thread state
   std::deque<Request> pending;
-  std::map<uint32_t, Request> active;  // batch job id -> request
+  std::map<uint32_t, Request> active;  // batch job id -> request (wavefront)
+  struct Launch {
+    cg_vbatch* vb = nullptr;
+    std::vector<Request> reqs;
+  };
+  std::deque<Launch> launches;  // in flight (kLaunches)
+  Clock::time_point busy_since;
 
   ~Impl() {
     {
@@ -198,7 +204,112 @@ struct Server::Impl {
     r.done_alone->set_value(finish(r, h));
   }
 
+  // kLaunches: admission windows become persistent launches of the per-pixel
+  // kernel (cg_view_batch_launch), up to max_launches_in_flight at once.
+  void loop_launches() {
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        if (pending.empty() && launches.empty())
+          cv_in.wait(lk, [&] { return stop || !incoming.empty(); });
+        else if (pending.empty() || launches.size() >= conf.max_launches_in_flight)
+          cv_in.wait_for(lk, std::chrono::microseconds(200), [&] { return stop || !incoming.empty(); });
+        if (stop && incoming.empty() && pending.empty() && launches.empty()) return;
+        while (!incoming.empty()) {
+          pending.push_back(std::move(incoming.front()));
+          incoming.pop_front();
+        }
+        if (stop && !error.empty()) return;
+      }
+      // finished launches, in any order
+      std::vector<RenderResult> out;
+      for (auto it = launches.begin(); it != launches.end();) {
+        int done = 0;
+        if (cg_view_batch_query(it->vb, &done) != CG_OK) {
+          std::lock_guard<std::mutex> g(mu);
+          fail_locked(last());
+          return;
+        }
+        if (!done) {
+          ++it;
+          continue;
+        }
+        cg_view_batch_release(it->vb);
+        for (Request& r : it->reqs) {
+          const size_t px = static_cast<size_t>(r.req.conf.render_width) * r.req.conf.render_height;
+          std::vector<cg_history_pixel> h(px);
+          if (cg_memcpy_d2h(env, h.data(), r.out, px * sizeof(cg_history_pixel)) != CG_OK) {
+            std::lock_guard<std::mutex> g(mu);
+            fail_locked(last());
+            return;
+          }
+          arena.release(r.out);
+          out.push_back(finish(r, h));
+        }
+        it = launches.erase(it);
+      }
+      if (!out.empty()) {
+        cg_stats st{};
+        cg_stats_read(env, &st);  // cumulative over every launch so far
+        std::lock_guard<std::mutex> g(mu);
+        stats.rays = st.rays + st.shadow_rays;
+        if (launches.empty()) stats.gpu_busy_s += seconds_since(busy_since, Clock::now());
+        for (auto& r : out) {
+          ++stats.completed;
+          results.push_back(std::move(r));
+        }
+        cv_out.notify_all();
+      }
+      // requests for the plain kernel alone, when nothing is in flight
+      if (launches.empty()) {
+        for (auto it = pending.begin(); it != pending.end();) {
+          if (!it->alone) {
+            ++it;
+            continue;
+          }
+          run_alone(*it);
+          it = pending.erase(it);
+        }
+      }
+      if (pending.empty() || launches.size() >= conf.max_launches_in_flight) continue;
+      // the next launch: every waiting request whose output fits, in arrival order
+      Launch L;
+      std::vector<cg_job> jobs;
+      for (auto it = pending.begin(); it != pending.end();) {
+        if (it->alone) {
+          ++it;
+          continue;
+        }
+        const size_t px = static_cast<size_t>(it->req.conf.render_width) * it->req.conf.render_height;
+        char* o = arena.alloc(px * sizeof(cg_history_pixel));
+        if (!o) {
+          ++it;
+          continue;
+        }
+        it->out = o;
+        it->admitted = Clock::now();
+        jobs.push_back(job_of(*it));
+        L.reqs.push_back(std::move(*it));
+        it = pending.erase(it);
+      }
+      if (jobs.empty()) continue;
+      if (cg_view_batch_launch(env, jobs.data(), static_cast<uint32_t>(jobs.size()), &L.vb) != CG_OK) {
+        std::lock_guard<std::mutex> g(mu);
+        fail_locked(last());
+        return;
+      }
+      if (launches.empty()) busy_since = Clock::now();
+      launches.push_back(std::move(L));
+      std::lock_guard<std::mutex> g(mu);
+      stats.admitted += jobs.size();
+    }
+  }
+
   void loop() {
+    if (conf.engine == ServeConf::Engine::kLaunches) {
+      loop_launches();
+      return;
+    }
     std::vector<uint32_t> done(4096);
     for (;;) {
       {
@@ -314,17 +425,26 @@ expected<std::unique_ptr<Server>, Error> Server::Create(const ServeConf& conf, c
     cg_scene_footprint(ds, nullptr, &depth);
     stack = std::max(stack, depth);
   }
-  const size_t out_bytes = static_cast<size_t>(static_cast<double>(conf.hbm_budget_bytes) * conf.output_fraction);
-  const size_t slot_bytes = 64 + 32 + 4 + 8 + 128 * static_cast<size_t>(std::max(1u, conf.max_trace_depth - 1));
-  const size_t slots = (conf.hbm_budget_bytes - out_bytes) / slot_bytes;
-  if (slots < 64) return unexpected<Error>(Error{"Server: budget too small"});
+  const bool wavefront = conf.engine == ServeConf::Engine::kWavefront;
+  const size_t out_bytes =
+      wavefront ? static_cast<size_t>(static_cast<double>(conf.hbm_budget_bytes) * conf.output_fraction)
+                : conf.hbm_budget_bytes;
   if (cg_device_alloc(impl->env, out_bytes, &impl->arena_mem) != CG_OK) return unexpected<Error>(Error{impl->last()});
   impl->arena.reset(static_cast<char*>(impl->arena_mem), out_bytes);
   impl->out_bytes = out_bytes;
-  impl->slot_capacity = std::min<size_t>(slots, 1u << 28) & ~size_t{63};
-  cg_batch_conf bc{static_cast<uint32_t>(std::min<size_t>(slots, 1u << 28)), conf.max_trace_depth, conf.rounds_per_launch,
-                   1u, 1024u, std::max(stack, 16u), conf.block_size, conf.short_stack};
-  if (cg_batch_create(impl->env, &bc, &impl->batch) != CG_OK) return unexpected<Error>(Error{impl->last()});
+  impl->slot_capacity = ~uint64_t{0};
+  if (wavefront) {
+    const size_t slot_bytes = 64 + 32 + 4 + 8 + 128 * static_cast<size_t>(std::max(1u, conf.max_trace_depth - 1));
+    const size_t slots = (conf.hbm_budget_bytes - out_bytes) / slot_bytes;
+    if (slots < 64) return unexpected<Error>(Error{"Server: budget too small"});
+    impl->slot_capacity = std::min<size_t>(slots, 1u << 28) & ~size_t{63};
+    cg_batch_conf bc{static_cast<uint32_t>(std::min<size_t>(slots, 1u << 28)), conf.max_trace_depth,
+                     conf.rounds_per_launch, 1u, 1024u, std::max(stack, 16u), conf.block_size, conf.short_stack};
+    if (cg_batch_create(impl->env, &bc, &impl->batch) != CG_OK) return unexpected<Error>(Error{impl->last()});
+  } else {
+    if (conf.max_launches_in_flight == 0) return unexpected<Error>(Error{"Server: max_launches_in_flight must be > 0"});
+    cg_env_set_counting(impl->env, 1);
+  }
   Impl* p = impl.get();
   impl->worker = std::thread([p] { p->loop(); });
   return std::unique_ptr<Server>(new Server(std::move(impl)));
@@ -428,12 +548,14 @@ typedef struct cgh_serve_stats {
 
 const char* cgh_serve_last_error(void) { return t_serve_err.c_str(); }
 
-int cgh_server_create(const cgh_scene* const* scenes, uint32_t n, uint64_t budget_bytes, double output_fraction,
-                      uint32_t max_trace_depth, uint32_t rounds_per_launch, uint32_t block_size, uint32_t short_stack,
-                      cgh_server** out) {
+int cgh_server_create(const cgh_scene* const* scenes, uint32_t n, uint32_t engine, uint64_t budget_bytes,
+                      double output_fraction, uint32_t max_trace_depth, uint32_t in_flight, uint32_t rounds_per_launch,
+                      uint32_t block_size, uint32_t short_stack, cgh_server** out) {
   std::vector<const Scene*> sc;
   for (uint32_t i = 0; i < n; ++i) sc.push_back(reinterpret_cast<const Scene*>(scenes[i]));
   ServeConf c;
+  c.engine = engine == 1 ? ServeConf::Engine::kWavefront : ServeConf::Engine::kLaunches;
+  c.max_launches_in_flight = in_flight;
   c.hbm_budget_bytes = budget_bytes;
   c.output_fraction = output_fraction;
   c.max_trace_depth = max_trace_depth;

@@ -21,12 +21,19 @@
 namespace crystal::graphics {
 
 struct ServeConf {
-  std::size_t hbm_budget_bytes = std::size_t{16} << 30;  // slot pool + request outputs
-  double output_fraction = 0.25;                         // of the budget kept for request outputs
+  // kLaunches: each admission window becomes one persistent launch of the
+  // per-pixel kernel over the admitted requests' tiles, several in flight on
+  // their own streams.  kWavefront: requests join the wavefront rounds of one
+  // batch engine (per-request completion, path state in a slot pool).
+  enum class Engine { kLaunches, kWavefront };
+  Engine engine = Engine::kLaunches;
+  std::size_t hbm_budget_bytes = std::size_t{16} << 30;  // request outputs (+ the slot pool, wavefront)
+  double output_fraction = 0.25;                         // wavefront: share of the budget for outputs
   std::uint32_t max_trace_depth = 8;
-  std::uint32_t rounds_per_launch = 16;
-  std::uint32_t block_size = 0;   // 0: default
-  std::uint32_t short_stack = 0;  // 0: default
+  std::uint32_t max_launches_in_flight = 4;              // kLaunches
+  std::uint32_t rounds_per_launch = 16;                  // kWavefront
+  std::uint32_t block_size = 0;   // kWavefront kernel shape, 0: default
+  std::uint32_t short_stack = 0;
 };
 
 struct RenderRequest {

@@ -12,7 +12,8 @@ from crystalgraphics_amd import workloads
 pytestmark = pytest.mark.gpu
 
 
-def test_serve_results_equal_alone(cg):
+@pytest.mark.parametrize("engine", [0, 1])
+def test_serve_results_equal_alone(cg, engine):
     scenes = [cg.HostScene.generate("spheres", seed=4, params=[6]),
               cg.HostScene.generate("c1", seed=3, params=[0.125])]
     rng = np.random.default_rng(5)
@@ -22,7 +23,7 @@ def test_serve_results_equal_alone(cg):
         pos, d = workloads.orbit_camera(rng, 3.0 if k % 2 else 12.0)
         cam = cg.Camera.make(pos, d, (0.5 * w / h, 0.5))
         reqs.append((k % 2, cam, cg.Conf.make(w, h, [1, 2, 4][k % 3], [2, 5, 8][k % 3])))
-    with cg.Server(scenes, budget_bytes=256 << 20, max_trace_depth=8, rounds_per_launch=4) as srv:
+    with cg.Server(scenes, budget_bytes=256 << 20, engine=engine, max_trace_depth=8, rounds_per_launch=4) as srv:
         ids = {}
         for i, (s, cam, conf) in enumerate(reqs):
             ids[srv.submit(s, cam, conf)] = i
@@ -45,7 +46,7 @@ def test_serve_results_equal_alone(cg):
 
 def test_serve_rejects_oversized_request(cg):
     scenes = [cg.HostScene.generate("spheres", seed=4, params=[4])]
-    with cg.Server(scenes, budget_bytes=16 << 20, max_trace_depth=4) as srv:
+    with cg.Server(scenes, budget_bytes=16 << 20, engine=1, max_trace_depth=4) as srv:
         with pytest.raises(cg.NativeError, match="budget"):
             srv.submit(0, scenes[0].describe()["camera"], cg.Conf.make(1920, 1080, 1, 2))
         with pytest.raises(cg.NativeError, match="max_trace_depth"):
