@@ -162,7 +162,7 @@ CG_HD TriT tri_test(f3 o, f3 d, f3 v0, f3 e1, f3 e2) {
 // ---------------------------------------------------------------------------
 // Scene view used by the kernels.
 struct DevScene {
-  const Node4* bvh;        // scene BVH, 4-wide
+  const Node4Q* bvh;       // scene BVH, 4-wide, quantised boxes
   const PairNode* nodes;   // emitter BVH
   const TriRec* tris;
   const InstRec* insts;
@@ -249,7 +249,7 @@ CG_HD void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
 // The scene pointers a traversal reads (per lane: lanes of one wave may
 // trace rays of different scenes).
 struct TravScene {
-  const Node4* bvh;
+  const Node4Q* bvh;
   const TriRec* tris;
   const InstRec* insts;
   uint32_t root;
@@ -297,22 +297,29 @@ CG_HD bool trav_step(const TravScene& S, TravRay& r, const TStack& stk, TravStat
   const float kInf = __builtin_inff();
   const uint32_t ref = r.ref;
   if ((ref & (CG_REF_LEAF | CG_REF_INST)) == 0u) {
-    const float4* q = reinterpret_cast<const float4*>(S.bvh + ref);
-    const float4 lx = q[0], ly = q[1], lz = q[2], hx = q[3], hy = q[4], hz = q[5];
-    const uint4 rr = reinterpret_cast<const uint4*>(q)[6];
+    // 64-B quantised node: origin + exponents, 8-bit child bounds, refs.
+    const uint4* q = reinterpret_cast<const uint4*>(S.bvh + ref);
+    const uint4 w0 = q[0], w1 = q[1], w2 = q[2], w3 = q[3];
     st.nodes++;
+    const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
+    const float sx = __uint_as_float((w0.w & 0xFFu) << 23), sy = __uint_as_float(((w0.w >> 8) & 0xFFu) << 23),
+                sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23);
+    // qlo x/y/z = w1.x/y/z, qhi x/y/z = w1.w, w2.x, w2.y; byte i = child i
+#define CG_QB(w, i) static_cast<float>(((w) >> (8 * (i))) & 0xFFu)
+#define CG_QBOX(i, t)                                                                                  \
+  {                                                                                                    \
+    const float b[3] = {ox + CG_QB(w1.x, i) * sx, oy + CG_QB(w1.y, i) * sy, oz + CG_QB(w1.z, i) * sz}; \
+    const float e[3] = {ox + CG_QB(w1.w, i) * sx, oy + CG_QB(w2.x, i) * sy, oz + CG_QB(w2.y, i) * sz}; \
+    t = box_enter(r.o, r.inv, b, e);                                                                   \
+  }
     float t0, t1, t2, t3;
-    uint32_t r0 = rr.x, r1 = rr.y, r2 = rr.z, r3 = rr.w;
-    {
-      const float b0[3] = {lx.x, ly.x, lz.x}, e0[3] = {hx.x, hy.x, hz.x};
-      const float b1[3] = {lx.y, ly.y, lz.y}, e1[3] = {hx.y, hy.y, hz.y};
-      const float b2[3] = {lx.z, ly.z, lz.z}, e2[3] = {hx.z, hy.z, hz.z};
-      const float b3[3] = {lx.w, ly.w, lz.w}, e3[3] = {hx.w, hy.w, hz.w};
-      t0 = box_enter(r.o, r.inv, b0, e0);
-      t1 = box_enter(r.o, r.inv, b1, e1);
-      t2 = box_enter(r.o, r.inv, b2, e2);
-      t3 = box_enter(r.o, r.inv, b3, e3);
-    }
+    uint32_t r0 = w2.z, r1 = w2.w, r2 = w3.x, r3 = w3.y;
+    CG_QBOX(0, t0)
+    CG_QBOX(1, t1)
+    CG_QBOX(2, t2)
+    CG_QBOX(3, t3)
+#undef CG_QBOX
+#undef CG_QB
     t0 = (t0 <= r.best.t && r0 != CG_REF_NONE) ? t0 : kInf;
     t1 = (t1 <= r.best.t && r1 != CG_REF_NONE) ? t1 : kInf;
     t2 = (t2 <= r.best.t && r2 != CG_REF_NONE) ? t2 : kInf;

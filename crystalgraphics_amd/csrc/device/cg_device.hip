@@ -752,7 +752,7 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
   auto up = [&](int r) {
     if (r != CG_OK && rc == CG_OK) rc = r;
   };
-  up(upload_vec(env, s->bvh, L.bvh, total));
+  up(upload_vec(env, s->bvh, L.bvhq, total));
   up(upload_vec(env, s->nodes, L.nodes, total));
   up(upload_vec(env, s->tris, L.tris, total));
   up(upload_vec(env, s->insts, L.insts, total));
@@ -773,7 +773,7 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
     return rc;
   }
   DevScene& v = s->view;
-  v.bvh = static_cast<const Node4*>(s->bvh.p);
+  v.bvh = static_cast<const Node4Q*>(s->bvh.p);
   v.nodes = static_cast<const PairNode*>(s->nodes.p);
   v.tris = static_cast<const TriRec*>(s->tris.p);
   v.insts = static_cast<const InstRec*>(s->insts.p);
@@ -795,6 +795,20 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
   s->bytes = total;
   env->scenes.push_back(s);
   *out = s;
+  return CG_OK;
+}
+
+int cg_layout_check(const cg_scene_data* sd, int accel, uint64_t* nodes, uint32_t* stack_depth) {
+  if (!sd) return fail(nullptr, CG_ERR_INVALID, "cg_layout_check: null scene data");
+  Layout L;
+  std::string err;
+  int rc = build_layout(sd, L, err, accel);
+  if (rc != CG_OK) return fail(nullptr, rc, "%s", err.c_str());
+  for (size_t i = 0; i < L.bvh.size(); ++i)
+    if (!quantized_contains(L.bvh[i], L.bvhq[i]))
+      return fail(nullptr, CG_ERR_INVALID, "quantised node %zu does not contain its children's boxes", i);
+  if (nodes) *nodes = L.bvh.size();
+  if (stack_depth) *stack_depth = L.stack_depth;
   return CG_OK;
 }
 

@@ -4,8 +4,10 @@
 // Every record is a whole number of 16-byte vectors and either 64 B or a
 // multiple of 16 B, so one traversal step is a handful of dwordx4 loads from
 // one or two 128-byte lines:
-//   Node4    128 B  one 4-wide BVH node (one 128-byte line): the boxes of up
-//                   to four children in SoA order, their refs
+//   Node4    128 B  one 4-wide BVH node: the boxes of up to four children in
+//                   SoA order, their refs (build time)
+//   Node4Q    64 B  the same node with quantised child boxes (what the
+//                   kernels traverse)
 //   PairNode  64 B  both child boxes of one interior node of the binary
 //                   emitter BVH + child refs
 //   TriRec    48 B  v0, e1 = v1 - v0, e2 = v2 - v0, source triangle id (and,
@@ -41,6 +43,19 @@ struct __attribute__((aligned(16))) Node4 {
   float hix[4], hiy[4], hiz[4];  // hi = -inf and ref CG_REF_NONE
   uint32_t ref[4];
   uint32_t _pad[4];
+};
+
+// Quantised 4-wide node, 64 B (half a line): child boxes as 8-bit offsets on
+// a per-axis power-of-two grid from the node's lower corner, rounded outward
+// (decoded lo <= true lo, decoded hi >= true hi, checked at build time with
+// the decoder's own float operations).
+struct __attribute__((aligned(16))) Node4Q {
+  float origin[3];
+  uint32_t exps;             // bits 0-7, 8-15, 16-23: biased exponents of the x, y, z grid steps
+  uint8_t qlo[3][4];         // [axis][child]
+  uint8_t qhi[3][4];
+  uint32_t ref[4];
+  uint32_t _pad[2];
 };
 
 struct __attribute__((aligned(16))) PairNode {
@@ -113,6 +128,7 @@ struct __attribute__((aligned(16))) WlLeaf {
 };
 
 static_assert(sizeof(Node4) == 128, "Node4");
+static_assert(sizeof(Node4Q) == 64, "Node4Q");
 static_assert(sizeof(PairNode) == 64, "PairNode");
 static_assert(sizeof(TriRec) == 48, "TriRec");
 static_assert(sizeof(InstRec) == 80, "InstRec");

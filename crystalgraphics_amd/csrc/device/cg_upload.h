@@ -132,7 +132,8 @@ inline void box_of(const float* lb, const float* ub, float* lo, float* hi) {
 }
 
 struct Layout {
-  std::vector<Node4> bvh;       // scene BVH (TLAS and BLASes), 4-wide
+  std::vector<Node4> bvh;       // scene BVH (TLAS and BLASes), 4-wide, exact boxes
+  std::vector<Node4Q> bvhq;     // the same nodes with quantised boxes (what is uploaded)
   std::vector<PairNode> nodes;  // binary trees while building; then the emitter BVH
   std::vector<TriRec> tris;
   std::vector<InstRec> insts;
@@ -914,14 +915,104 @@ inline void collapse_scene_bvh(Layout& L) {
   L.nodes.shrink_to_fit();
 }
 
+// ---------------------------------------------------------------------------
+// Quantised nodes.  The decoder (trav_step) computes a bound as
+// origin + float(q) * step with step = 2^(e - 127): one exact product and one
+// rounded sum.  The encoder picks per axis the smallest step whose 255th grid
+// line reaches the node's upper bound, then per child the largest q whose
+// decoded value is <= the child's lower bound and the smallest q whose decoded
+// value is >= its upper bound, evaluating the decoder's own expression.
+inline float q_step(uint32_t biased) {
+  const uint32_t bits = biased << 23;
+  float f;
+  std::memcpy(&f, &bits, sizeof f);
+  return f;
+}
+inline float q_decode(float origin, uint32_t q, float step) { return origin + static_cast<float>(q) * step; }
+
+inline Node4Q quantize_node(const Node4& n) {
+  Node4Q out;
+  std::memset(&out, 0, sizeof out);
+  const float* lo[3] = {n.lox, n.loy, n.loz};
+  const float* hi[3] = {n.hix, n.hiy, n.hiz};
+  int used[4];
+  int nu = 0;
+  for (int i = 0; i < 4; ++i) {
+    out.ref[i] = n.ref[i];
+    if (n.ref[i] != CG_REF_NONE) used[nu++] = i;
+  }
+  uint32_t exps = 0;
+  for (int a = 0; a < 3; ++a) {
+    float blo = __builtin_inff(), bhi = -__builtin_inff();
+    for (int k = 0; k < nu; ++k) {
+      blo = std::min(blo, lo[a][used[k]]);
+      bhi = std::max(bhi, hi[a][used[k]]);
+    }
+    if (nu == 0) blo = bhi = 0.0f;
+    out.origin[a] = blo;
+    const double extent = static_cast<double>(bhi) - static_cast<double>(blo);
+    int be = 1;
+    if (extent > 0.0) be = std::max(1, static_cast<int>(std::floor(std::log2(extent / 255.0))) + 127 - 1);
+    while (be < 254 && q_decode(blo, 255u, q_step(static_cast<uint32_t>(be))) < bhi) ++be;
+    const float step = q_step(static_cast<uint32_t>(be));
+    exps |= static_cast<uint32_t>(be) << (8 * a);
+    for (int k = 0; k < nu; ++k) {
+      const int i = used[k];
+      double fl = std::floor((static_cast<double>(lo[a][i]) - blo) / step);
+      uint32_t ql = static_cast<uint32_t>(std::min(255.0, std::max(0.0, fl)));
+      while (ql > 0u && q_decode(blo, ql, step) > lo[a][i]) --ql;
+      double fh = std::ceil((static_cast<double>(hi[a][i]) - blo) / step);
+      uint32_t qh = static_cast<uint32_t>(std::min(255.0, std::max(0.0, fh)));
+      while (qh < 255u && q_decode(blo, qh, step) < hi[a][i]) ++qh;
+      out.qlo[a][i] = static_cast<uint8_t>(ql);
+      out.qhi[a][i] = static_cast<uint8_t>(qh);
+    }
+  }
+  out.exps = exps;
+  return out;
+}
+
+// Every decoded child box contains the exact one (false: the node that fails).
+inline bool quantized_contains(const Node4& n, const Node4Q& q) {
+  const float* lo[3] = {n.lox, n.loy, n.loz};
+  const float* hi[3] = {n.hix, n.hiy, n.hiz};
+  for (int i = 0; i < 4; ++i) {
+    if (n.ref[i] != q.ref[i]) return false;
+    if (n.ref[i] == CG_REF_NONE) continue;
+    for (int a = 0; a < 3; ++a) {
+      const float step = q_step((q.exps >> (8 * a)) & 0xFFu);
+      if (!(q_decode(q.origin[a], q.qlo[a][i], step) <= lo[a][i])) return false;
+      if (!(q_decode(q.origin[a], q.qhi[a][i], step) >= hi[a][i])) return false;
+    }
+  }
+  return true;
+}
+
+inline void quantize_scene_bvh(Layout& L) {
+  L.bvhq.resize(L.bvh.size());
+  const size_t n = L.bvh.size();
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const size_t chunk = (n + hw - 1) / hw;
+  std::vector<std::thread> pool;
+  for (unsigned t = 0; t < hw; ++t) {
+    const size_t b = t * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    pool.emplace_back([&L, b, e] {
+      for (size_t i = b; i < e; ++i) L.bvhq[i] = quantize_node(L.bvh[i]);
+    });
+  }
+  for (auto& th : pool) th.join();
+}
+
 // accel 0: the reference's trees; accel 1: rebuilt (SAH, merged world BLAS).
-// Either is traversed as a 4-wide BVH.
+// Either is traversed as a 4-wide BVH of quantised nodes.
 inline int build_layout(const cg_scene_data* sd, Layout& L, std::string& err, int accel = 1) {
   int rc = build_common(sd, L, err);
   if (rc != CG_OK) return rc;
   rc = accel == 0 ? build_reference_accel(sd, L, err) : build_sah_accel(sd, L, err);
   if (rc != CG_OK) return rc;
   collapse_scene_bvh(L);
+  quantize_scene_bvh(L);
   build_emitter_bvh(L);
   return CG_OK;
 }

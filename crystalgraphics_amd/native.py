@@ -211,6 +211,7 @@ def device() -> C.CDLL:
         _sig(d, "cg_scene_upload", I32, P, C.POINTER(SceneDataC), C.POINTER(P))
         _sig(d, "cg_scene_release", I32, P, P)
         _sig(d, "cg_scene_footprint", I32, P, C.POINTER(C.c_uint64), C.POINTER(U32))
+        _sig(d, "cg_layout_check", I32, C.POINTER(SceneDataC), I32, C.POINTER(C.c_uint64), C.POINTER(U32))
         _sig(d, "cg_view", I32, P, P, C.POINTER(Camera), U32)
         _sig(d, "cg_read_target", I32, P, P, P)
         _sig(d, "cg_read_path_records", I32, P, P, P)

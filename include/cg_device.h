@@ -90,6 +90,10 @@ int cg_stats_reset(cg_env* env);
  * Materials with textures give CG_ERR_UNSUPPORTED. */
 int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out);
 int cg_scene_release(cg_env* env, cg_scene* scene);
+/* Host-only check of the device layout cg_scene_upload would build (no GPU
+ * needed): every quantised BVH node's decoded child boxes contain the exact
+ * ones.  Reports the node count and the traversal stack bound. */
+int cg_layout_check(const cg_scene_data* sd, int accel, uint64_t* nodes, uint32_t* stack_depth);
 /* Device bytes held by a scene, and the traversal stack depth it needs. */
 int cg_scene_footprint(const cg_scene* scene, uint64_t* bytes, uint32_t* stack_depth);
 

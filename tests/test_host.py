@@ -153,3 +153,18 @@ def test_ray_batch_halves(cg):
     assert np.allclose(coherent["org"], coherent["org"][0])
     assert np.allclose(np.linalg.norm(r["dir"], axis=1), 1, atol=1e-5)
     assert np.ptp(incoherent["org"], axis=0).min() > 0.1
+
+
+@pytest.mark.parametrize("name,params,accel", [("c0", [], 1), ("c0", [], 0), ("c1", [0.25], 1), ("spheres", [8], 1),
+                                               ("c2", [200000], 1)])
+def test_quantised_nodes_contain_exact_boxes(cg, name, params, accel):
+    """The device layout's 64-B quantised nodes decode (with the kernels' own
+    float operations) to boxes that contain the exact child boxes, so the
+    traversal visits a superset of the exact tree's nodes."""
+    from crystalgraphics_amd import native
+    s = cg.HostScene.generate(name, seed=2, params=params)
+    sd = s.scene_data()
+    n, depth = C.c_uint64(), C.c_uint32()
+    rc = native.device().cg_layout_check(C.byref(sd.c), accel, C.byref(n), C.byref(depth))
+    assert rc == 0, native.device().cg_last_error()
+    assert n.value > 0 and 1 <= depth.value <= 128
