@@ -29,10 +29,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of one traversal query in the device layout: ray in (32)
-# + hit out (16) + 64 B per quantised 4-wide BVH node visited + 48 B per triangle
+# + hit out (16) + 128 B per quantised 8-wide BVH node visited + 48 B per triangle
 # record tested + 80 B per instance record entered.
 RAY_IO_BYTES = 32 + 16
-NODE_BYTES, TRI_BYTES, INST_BYTES = 64, 48, 80
+NODE_BYTES, TRI_BYTES, INST_BYTES = 128, 48, 80
 
 
 class HipEvents:

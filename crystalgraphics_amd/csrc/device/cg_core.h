@@ -162,7 +162,7 @@ CG_HD TriT tri_test(f3 o, f3 d, f3 v0, f3 e1, f3 e2) {
 // ---------------------------------------------------------------------------
 // Scene view used by the kernels.
 struct DevScene {
-  const Node4Q* bvh;       // scene BVH, 4-wide, quantised boxes
+  const Node8Q* bvh;       // scene BVH, 8-wide, quantised boxes
   const PairNode* nodes;   // emitter BVH
   const TriRec* tris;
   const InstRec* insts;
@@ -249,7 +249,7 @@ CG_HD void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
 // The scene pointers a traversal reads (per lane: lanes of one wave may
 // trace rays of different scenes).
 struct TravScene {
-  const Node4Q* bvh;
+  const Node8Q* bvh;
   const TriRec* tris;
   const InstRec* insts;
   uint32_t root;
@@ -297,43 +297,52 @@ CG_HD bool trav_step(const TravScene& S, TravRay& r, const TStack& stk, TravStat
   const float kInf = __builtin_inff();
   const uint32_t ref = r.ref;
   if ((ref & (CG_REF_LEAF | CG_REF_INST)) == 0u) {
-    // 64-B quantised node: origin + exponents, 8-bit child bounds, refs.
+    // 128-B quantised node: origin + exponents, 8-bit child bounds
+    // (lo x/y/z, hi x/y/z; 8 bytes each, byte i = child i), refs.
     const uint4* q = reinterpret_cast<const uint4*>(S.bvh + ref);
-    const uint4 w0 = q[0], w1 = q[1], w2 = q[2], w3 = q[3];
+    const uint4 w0 = q[0], w1 = q[1], w2 = q[2], w3 = q[3], w4 = q[4], w5 = q[5];
     st.nodes++;
     const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
     const float sx = __uint_as_float((w0.w & 0xFFu) << 23), sy = __uint_as_float(((w0.w >> 8) & 0xFFu) << 23),
                 sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23);
-    // qlo x/y/z = w1.x/y/z, qhi x/y/z = w1.w, w2.x, w2.y; byte i = child i
-#define CG_QB(w, i) static_cast<float>(((w) >> (8 * (i))) & 0xFFu)
-#define CG_QBOX(i, t)                                                                                  \
-  {                                                                                                    \
-    const float b[3] = {ox + CG_QB(w1.x, i) * sx, oy + CG_QB(w1.y, i) * sy, oz + CG_QB(w1.z, i) * sz}; \
-    const float e[3] = {ox + CG_QB(w1.w, i) * sx, oy + CG_QB(w2.x, i) * sy, oz + CG_QB(w2.y, i) * sz}; \
-    t = box_enter(r.o, r.inv, b, e);                                                                   \
-  }
-    float t0, t1, t2, t3;
-    uint32_t r0 = w2.z, r1 = w2.w, r2 = w3.x, r3 = w3.y;
-    CG_QBOX(0, t0)
-    CG_QBOX(1, t1)
-    CG_QBOX(2, t2)
-    CG_QBOX(3, t3)
-#undef CG_QBOX
+    float t[CG_WIDE];
+    uint32_t c[CG_WIDE] = {w4.x, w4.y, w4.z, w4.w, w5.x, w5.y, w5.z, w5.w};
+#define CG_QB(w, i) static_cast<float>(((w) >> (8 * ((i) & 3))) & 0xFFu)
+#pragma unroll
+    for (int i = 0; i < CG_WIDE; ++i) {
+      const uint32_t lx = i < 4 ? w1.x : w1.y, ly = i < 4 ? w1.z : w1.w, lz = i < 4 ? w2.x : w2.y;
+      const uint32_t hx = i < 4 ? w2.z : w2.w, hy = i < 4 ? w3.x : w3.y, hz = i < 4 ? w3.z : w3.w;
+      const float b[3] = {ox + CG_QB(lx, i) * sx, oy + CG_QB(ly, i) * sy, oz + CG_QB(lz, i) * sz};
+      const float e[3] = {ox + CG_QB(hx, i) * sx, oy + CG_QB(hy, i) * sy, oz + CG_QB(hz, i) * sz};
+      const float tn = box_enter(r.o, r.inv, b, e);
+      t[i] = (tn <= r.best.t && c[i] != CG_REF_NONE) ? tn : kInf;
+    }
 #undef CG_QB
-    t0 = (t0 <= r.best.t && r0 != CG_REF_NONE) ? t0 : kInf;
-    t1 = (t1 <= r.best.t && r1 != CG_REF_NONE) ? t1 : kInf;
-    t2 = (t2 <= r.best.t && r2 != CG_REF_NONE) ? t2 : kInf;
-    t3 = (t3 <= r.best.t && r3 != CG_REF_NONE) ? t3 : kInf;
-    cswap(t0, r0, t1, r1);
-    cswap(t2, r2, t3, r3);
-    cswap(t0, r0, t2, r2);
-    cswap(t1, r1, t3, r3);
-    cswap(t1, r1, t2, r2);
-    if (t3 < kInf) stk.push(r.sp, r3, t3);
-    if (t2 < kInf) stk.push(r.sp, r2, t2);
-    if (t1 < kInf) stk.push(r.sp, r1, t1);
-    if (t0 < kInf) {
-      r.ref = r0;
+    // nearest first: an 8-input sorting network (19 compare-exchanges)
+    cswap(t[0], c[0], t[2], c[2]);
+    cswap(t[1], c[1], t[3], c[3]);
+    cswap(t[4], c[4], t[6], c[6]);
+    cswap(t[5], c[5], t[7], c[7]);
+    cswap(t[0], c[0], t[4], c[4]);
+    cswap(t[1], c[1], t[5], c[5]);
+    cswap(t[2], c[2], t[6], c[6]);
+    cswap(t[3], c[3], t[7], c[7]);
+    cswap(t[0], c[0], t[1], c[1]);
+    cswap(t[2], c[2], t[3], c[3]);
+    cswap(t[4], c[4], t[5], c[5]);
+    cswap(t[6], c[6], t[7], c[7]);
+    cswap(t[2], c[2], t[4], c[4]);
+    cswap(t[3], c[3], t[5], c[5]);
+    cswap(t[1], c[1], t[4], c[4]);
+    cswap(t[3], c[3], t[6], c[6]);
+    cswap(t[1], c[1], t[2], c[2]);
+    cswap(t[3], c[3], t[4], c[4]);
+    cswap(t[5], c[5], t[6], c[6]);
+#pragma unroll
+    for (int i = CG_WIDE - 1; i >= 1; --i)
+      if (t[i] < kInf) stk.push(r.sp, c[i], t[i]);
+    if (t[0] < kInf) {
+      r.ref = c[0];
       return false;
     }
   } else if ((ref & CG_REF_LEAF) == 0u) {

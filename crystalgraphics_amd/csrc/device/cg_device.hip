@@ -773,7 +773,7 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
     return rc;
   }
   DevScene& v = s->view;
-  v.bvh = static_cast<const Node4Q*>(s->bvh.p);
+  v.bvh = static_cast<const Node8Q*>(s->bvh.p);
   v.nodes = static_cast<const PairNode*>(s->nodes.p);
   v.tris = static_cast<const TriRec*>(s->tris.p);
   v.insts = static_cast<const InstRec*>(s->insts.p);

@@ -4,10 +4,8 @@
 // Every record is a whole number of 16-byte vectors and either 64 B or a
 // multiple of 16 B, so one traversal step is a handful of dwordx4 loads from
 // one or two 128-byte lines:
-//   Node4    128 B  one 4-wide BVH node: the boxes of up to four children in
-//                   SoA order, their refs (build time)
-//   Node4Q    64 B  the same node with quantised child boxes (what the
-//                   kernels traverse)
+//   Node8Q   128 B  one 8-wide BVH node (one line): quantised boxes of up
+//                   to eight children, their refs
 //   PairNode  64 B  both child boxes of one interior node of the binary
 //                   emitter BVH + child refs
 //   TriRec    48 B  v0, e1 = v1 - v0, e2 = v2 - v0, source triangle id (and,
@@ -22,7 +20,7 @@
 #include <stdint.h>
 
 // Child / root references (32 bits):
-//   00 | index       interior node (30-bit index: Node4 in the scene BVH,
+//   00 | index       interior node (30-bit index: Node8Q in the scene BVH,
 //                    PairNode in the emitter BVH)
 //   01 | index       TLAS leaf: instance index (30-bit)
 //   1 cc | slot      BLAS leaf: cc + 1 triangles (1-4) from slot (29-bit)
@@ -38,24 +36,24 @@
 #define CG_LEAF_MAX 4u
 #define CG_MAKE_LEAF(slot, count) (CG_REF_LEAF | (((count) - 1u) << CG_REF_COUNT_SHIFT) | (slot))
 
-struct __attribute__((aligned(16))) Node4 {
-  float lox[4], loy[4], loz[4];  // child boxes; an empty slot has lo = +inf,
-  float hix[4], hiy[4], hiz[4];  // hi = -inf and ref CG_REF_NONE
-  uint32_t ref[4];
-  uint32_t _pad[4];
+// Up to eight children: exact boxes (build time only).
+#define CG_WIDE 8
+struct Node8 {
+  float lo[3][CG_WIDE], hi[3][CG_WIDE];  // [axis][child]; an empty slot has ref CG_REF_NONE
+  uint32_t ref[CG_WIDE];
 };
 
-// Quantised 4-wide node, 64 B (half a line): child boxes as 8-bit offsets on
-// a per-axis power-of-two grid from the node's lower corner, rounded outward
-// (decoded lo <= true lo, decoded hi >= true hi, checked at build time with
-// the decoder's own float operations).
-struct __attribute__((aligned(16))) Node4Q {
+// The node the kernels traverse, 128 B (one line): child boxes as 8-bit
+// offsets on a per-axis power-of-two grid from the node's lower corner,
+// rounded outward (decoded lo <= true lo, decoded hi >= true hi, checked at
+// build time with the decoder's own float operations).
+struct __attribute__((aligned(16))) Node8Q {
   float origin[3];
-  uint32_t exps;             // bits 0-7, 8-15, 16-23: biased exponents of the x, y, z grid steps
-  uint8_t qlo[3][4];         // [axis][child]
-  uint8_t qhi[3][4];
-  uint32_t ref[4];
-  uint32_t _pad[2];
+  uint32_t exps;                // bits 0-7, 8-15, 16-23: biased exponents of the x, y, z grid steps
+  uint8_t qlo[3][CG_WIDE];      // [axis][child]
+  uint8_t qhi[3][CG_WIDE];
+  uint32_t ref[CG_WIDE];
+  uint32_t _pad[8];
 };
 
 struct __attribute__((aligned(16))) PairNode {
@@ -127,8 +125,7 @@ struct __attribute__((aligned(16))) WlLeaf {
   float _pad[3];
 };
 
-static_assert(sizeof(Node4) == 128, "Node4");
-static_assert(sizeof(Node4Q) == 64, "Node4Q");
+static_assert(sizeof(Node8Q) == 128, "Node8Q");
 static_assert(sizeof(PairNode) == 64, "PairNode");
 static_assert(sizeof(TriRec) == 48, "TriRec");
 static_assert(sizeof(InstRec) == 80, "InstRec");

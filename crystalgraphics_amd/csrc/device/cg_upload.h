@@ -132,8 +132,8 @@ inline void box_of(const float* lb, const float* ub, float* lo, float* hi) {
 }
 
 struct Layout {
-  std::vector<Node4> bvh;       // scene BVH (TLAS and BLASes), 4-wide, exact boxes
-  std::vector<Node4Q> bvhq;     // the same nodes with quantised boxes (what is uploaded)
+  std::vector<Node8> bvh;       // scene BVH (TLAS and BLASes), 8-wide, exact boxes
+  std::vector<Node8Q> bvhq;     // the same nodes with quantised boxes (what is uploaded)
   std::vector<PairNode> nodes;  // binary trees while building; then the emitter BVH
   std::vector<TriRec> tris;
   std::vector<InstRec> insts;
@@ -810,17 +810,17 @@ inline void build_emitter_bvh(Layout& L) {
 }
 
 // ---------------------------------------------------------------------------
-// 4-wide BVH from a binary one: each 4-wide node takes an interior binary
+// 8-wide BVH from a binary one: each wide node takes an interior binary
 // node's two children and repeatedly replaces the interior child of largest
-// surface area by its own two children, up to four.  Every child box is a
-// box of the binary tree, bit for bit, so each box test decides as it did in
-// the binary traversal; a traversal visits about half as many nodes, each one
-// 128-byte line.  Nodes are stored parent first, depth first.
-struct Collapse4 {
+// surface area by its own two children, up to eight.  Every child box is a
+// box of the binary tree, bit for bit.  A traversal visits about a third as
+// many nodes as the binary walk, each one 128-byte line.  Nodes are stored
+// parent first, depth first.
+struct CollapseWide {
   const std::vector<PairNode>& pairs;
-  std::vector<Node4>& out;
+  std::vector<Node8>& out;
   std::vector<uint32_t> map;
-  explicit Collapse4(const std::vector<PairNode>& p, std::vector<Node4>& o)
+  explicit CollapseWide(const std::vector<PairNode>& p, std::vector<Node8>& o)
       : pairs(p), out(o), map(p.size(), CG_REF_NONE) {}
   static bool interior(uint32_t r) { return r != CG_REF_NONE && (r & (CG_REF_LEAF | CG_REF_INST)) == 0u; }
   static float half_area(const float* lo, const float* hi) {
@@ -840,11 +840,11 @@ struct Collapse4 {
   uint32_t convert(uint32_t r) {
     if (!interior(r)) return r;
     if (map[r] != CG_REF_NONE) return map[r];
-    Child ch[4];
+    Child ch[CG_WIDE];
     int n = 2;
     ch[0] = left(pairs[r]);
     ch[1] = right(pairs[r]);
-    while (n < 4) {
+    while (n < CG_WIDE) {
       int best = -1;
       float ba = -1.0f;
       for (int i = 0; i < n; ++i)
@@ -863,18 +863,14 @@ struct Collapse4 {
     const uint32_t idx = static_cast<uint32_t>(out.size());
     map[r] = idx;
     out.emplace_back();
-    Node4 nd;
-    for (int i = 0; i < 4; ++i) {
+    Node8 nd;
+    for (int i = 0; i < CG_WIDE; ++i) {
       const bool used = i < n;
-      const float inf = __builtin_inff();
-      nd.lox[i] = used ? ch[i].lo[0] : inf;
-      nd.loy[i] = used ? ch[i].lo[1] : inf;
-      nd.loz[i] = used ? ch[i].lo[2] : inf;
-      nd.hix[i] = used ? ch[i].hi[0] : -inf;
-      nd.hiy[i] = used ? ch[i].hi[1] : -inf;
-      nd.hiz[i] = used ? ch[i].hi[2] : -inf;
+      for (int a = 0; a < 3; ++a) {
+        nd.lo[a][i] = used ? ch[i].lo[a] : __builtin_inff();
+        nd.hi[a][i] = used ? ch[i].hi[a] : -__builtin_inff();
+      }
       nd.ref[i] = CG_REF_NONE;
-      nd._pad[i] = 0u;
     }
     for (int i = 0; i < n; ++i) nd.ref[i] = convert(ch[i].ref);
     out[idx] = nd;
@@ -882,35 +878,35 @@ struct Collapse4 {
   }
 };
 
-// Deepest traversal stack of a 4-wide tree: a node with k children pushes at
+// Deepest traversal stack of a wide tree: a node with k children pushes at
 // most k - 1 of them and descends into the nearest; entering an instance
 // pushes one marker.
-inline uint32_t stack_bound4(const std::vector<Node4>& bvh, const std::vector<InstRec>& insts, uint32_t r,
-                             std::vector<uint32_t>& memo) {
+inline uint32_t stack_bound_wide(const std::vector<Node8>& bvh, const std::vector<InstRec>& insts, uint32_t r,
+                                 std::vector<uint32_t>& memo) {
   if (r == CG_REF_NONE || (r & CG_REF_LEAF) != 0u) return 0u;
   if ((r & CG_REF_INST) != 0u) {
     const InstRec& in = insts[r & CG_REF_INDEX_MASK];
-    return in.root == CG_REF_NONE ? 0u : 1u + stack_bound4(bvh, insts, in.root, memo);
+    return in.root == CG_REF_NONE ? 0u : 1u + stack_bound_wide(bvh, insts, in.root, memo);
   }
   if (memo[r] != ~0u) return memo[r];
-  const Node4& n = bvh[r];
+  const Node8& n = bvh[r];
   uint32_t k = 0, deepest = 0;
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < CG_WIDE; ++i)
     if (n.ref[i] != CG_REF_NONE) {
       ++k;
-      deepest = std::max(deepest, stack_bound4(bvh, insts, n.ref[i], memo));
+      deepest = std::max(deepest, stack_bound_wide(bvh, insts, n.ref[i], memo));
     }
   return memo[r] = (k > 0 ? k - 1u : 0u) + deepest;
 }
 
 inline void collapse_scene_bvh(Layout& L) {
   L.bvh.clear();
-  L.bvh.reserve(L.nodes.size() / 2 + 1);
-  Collapse4 c(L.nodes, L.bvh);
+  L.bvh.reserve(L.nodes.size() / 4 + 1);
+  CollapseWide c(L.nodes, L.bvh);
   for (InstRec& in : L.insts) in.root = c.convert(in.root);
   L.root = c.convert(L.root);
   std::vector<uint32_t> memo(L.bvh.size(), ~0u);
-  L.stack_depth = stack_bound4(L.bvh, L.insts, L.root, memo);
+  L.stack_depth = stack_bound_wide(L.bvh, L.insts, L.root, memo);
   L.nodes.clear();
   L.nodes.shrink_to_fit();
 }
@@ -930,14 +926,12 @@ inline float q_step(uint32_t biased) {
 }
 inline float q_decode(float origin, uint32_t q, float step) { return origin + static_cast<float>(q) * step; }
 
-inline Node4Q quantize_node(const Node4& n) {
-  Node4Q out;
+inline Node8Q quantize_node(const Node8& n) {
+  Node8Q out;
   std::memset(&out, 0, sizeof out);
-  const float* lo[3] = {n.lox, n.loy, n.loz};
-  const float* hi[3] = {n.hix, n.hiy, n.hiz};
-  int used[4];
+  int used[CG_WIDE];
   int nu = 0;
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < CG_WIDE; ++i) {
     out.ref[i] = n.ref[i];
     if (n.ref[i] != CG_REF_NONE) used[nu++] = i;
   }
@@ -945,8 +939,8 @@ inline Node4Q quantize_node(const Node4& n) {
   for (int a = 0; a < 3; ++a) {
     float blo = __builtin_inff(), bhi = -__builtin_inff();
     for (int k = 0; k < nu; ++k) {
-      blo = std::min(blo, lo[a][used[k]]);
-      bhi = std::max(bhi, hi[a][used[k]]);
+      blo = std::min(blo, n.lo[a][used[k]]);
+      bhi = std::max(bhi, n.hi[a][used[k]]);
     }
     if (nu == 0) blo = bhi = 0.0f;
     out.origin[a] = blo;
@@ -958,12 +952,12 @@ inline Node4Q quantize_node(const Node4& n) {
     exps |= static_cast<uint32_t>(be) << (8 * a);
     for (int k = 0; k < nu; ++k) {
       const int i = used[k];
-      double fl = std::floor((static_cast<double>(lo[a][i]) - blo) / step);
+      const double fl = std::floor((static_cast<double>(n.lo[a][i]) - blo) / step);
       uint32_t ql = static_cast<uint32_t>(std::min(255.0, std::max(0.0, fl)));
-      while (ql > 0u && q_decode(blo, ql, step) > lo[a][i]) --ql;
-      double fh = std::ceil((static_cast<double>(hi[a][i]) - blo) / step);
+      while (ql > 0u && q_decode(blo, ql, step) > n.lo[a][i]) --ql;
+      const double fh = std::ceil((static_cast<double>(n.hi[a][i]) - blo) / step);
       uint32_t qh = static_cast<uint32_t>(std::min(255.0, std::max(0.0, fh)));
-      while (qh < 255u && q_decode(blo, qh, step) < hi[a][i]) ++qh;
+      while (qh < 255u && q_decode(blo, qh, step) < n.hi[a][i]) ++qh;
       out.qlo[a][i] = static_cast<uint8_t>(ql);
       out.qhi[a][i] = static_cast<uint8_t>(qh);
     }
@@ -972,17 +966,15 @@ inline Node4Q quantize_node(const Node4& n) {
   return out;
 }
 
-// Every decoded child box contains the exact one (false: the node that fails).
-inline bool quantized_contains(const Node4& n, const Node4Q& q) {
-  const float* lo[3] = {n.lox, n.loy, n.loz};
-  const float* hi[3] = {n.hix, n.hiy, n.hiz};
-  for (int i = 0; i < 4; ++i) {
+// Every decoded child box contains the exact one.
+inline bool quantized_contains(const Node8& n, const Node8Q& q) {
+  for (int i = 0; i < CG_WIDE; ++i) {
     if (n.ref[i] != q.ref[i]) return false;
     if (n.ref[i] == CG_REF_NONE) continue;
     for (int a = 0; a < 3; ++a) {
       const float step = q_step((q.exps >> (8 * a)) & 0xFFu);
-      if (!(q_decode(q.origin[a], q.qlo[a][i], step) <= lo[a][i])) return false;
-      if (!(q_decode(q.origin[a], q.qhi[a][i], step) >= hi[a][i])) return false;
+      if (!(q_decode(q.origin[a], q.qlo[a][i], step) <= n.lo[a][i])) return false;
+      if (!(q_decode(q.origin[a], q.qhi[a][i], step) >= n.hi[a][i])) return false;
     }
   }
   return true;
@@ -1005,7 +997,7 @@ inline void quantize_scene_bvh(Layout& L) {
 }
 
 // accel 0: the reference's trees; accel 1: rebuilt (SAH, merged world BLAS).
-// Either is traversed as a 4-wide BVH of quantised nodes.
+// Either is traversed as an 8-wide BVH of quantised nodes.
 inline int build_layout(const cg_scene_data* sd, Layout& L, std::string& err, int accel = 1) {
   int rc = build_common(sd, L, err);
   if (rc != CG_OK) return rc;
