@@ -200,8 +200,14 @@ __device__ __forceinline__ TStack lane_stack(uint32_t* s_ref, float* s_t, const 
 // wave is idle the idle lanes take the next rays.  Lanes stay busy instead of
 // waiting for the slowest ray of the wave.  fetch(i, scene, o, d, tmax, any)
 // sets up ray i; done(i, ray) stores its result.
+// LDS stack entries per lane in the one-job per-pixel kernel.  One frame is
+// bound by its longest pixel chains (latency): more registers (3 waves per
+// SIMD) and a deeper LDS stack (24 x 8 B x 256 lanes, three blocks per CU)
+// shorten them.  The many-job kernel is bound by throughput and keeps 4 waves
+// per SIMD and 16 entries.
+constexpr int kViewShortStack = 24;
 constexpr int kRefillLanes = 16;
-constexpr int kStepsPerRefill = 8;
+constexpr int kStepsPerRefill = 4;
 template <int kMode, class Fetch, class Done>
 __device__ __forceinline__ void trace_refill(const WorkQueue& q, uint32_t n, const TStack& stk, TravStats& st,
                                              Fetch fetch, Done done) {
@@ -311,10 +317,10 @@ __global__ __launch_bounds__(kBlock, 5) void k_trace(DevScene S, const cg_ray* _
 // in a private array (MF frames = trace_depth - 1); the BVH stack in LDS
 // (overflow in device memory).
 template <int MF>
-__global__ __launch_bounds__(kBlock, 4) void k_view(ViewArgs A, WorkQueue q, StackArgs sa) {
-  __shared__ uint32_t s_ref[kShortStack * kBlock];
-  __shared__ float s_t[kShortStack * kBlock];
-  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kShortStack);
+__global__ __launch_bounds__(kBlock, 3) void k_view(ViewArgs A, WorkQueue q, StackArgs sa) {
+  __shared__ uint32_t s_ref[kViewShortStack * kBlock];
+  __shared__ float s_t[kViewShortStack * kBlock];
+  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kViewShortStack);
   const uint32_t tiles_x = (A.W + 7u) / 8u;
   TravStats st{0u, 0u, 0u};
   uint64_t nrays = 0, nshadow = 0;
@@ -542,11 +548,11 @@ WorkQueue reset_queue(cg_env* env, uint32_t items) {
 }
 // Overflow stack area for a launch of `grid` blocks on a scene whose
 // traversal stack is at most `depth` entries deep.
-int stack_args(cg_env* env, uint32_t grid, uint32_t depth, StackArgs& sa) {
+int stack_args(cg_env* env, uint32_t grid, uint32_t depth, StackArgs& sa, int short_n = kShortStack) {
   sa.ovf = nullptr;
   sa.ovf_cap = 0;
-  if (depth <= static_cast<uint32_t>(kShortStack)) return CG_OK;
-  const int cap = static_cast<int>(depth) - kShortStack;
+  if (depth <= static_cast<uint32_t>(short_n)) return CG_OK;
+  const int cap = static_cast<int>(depth) - short_n;
   const size_t bytes = static_cast<size_t>(grid) * kBlock * static_cast<size_t>(cap) * sizeof(uint2);
   if (env->ovf.bytes < bytes) {
     if (env->ovf.p) {
@@ -564,7 +570,7 @@ template <int MF>
 int launch_view_mf(cg_env* env, const ViewArgs& a, uint32_t tiles, uint32_t depth) {
   const uint32_t grid = resident_grid(k_view<MF>, env, (tiles + 3u) / 4u);
   StackArgs sa;
-  int rc = stack_args(env, grid, depth, sa);
+  int rc = stack_args(env, grid, depth, sa, kViewShortStack);
   if (rc != CG_OK) return rc;
   WorkQueue q = reset_queue(env, tiles);
   hipLaunchKernelGGL((k_view<MF>), dim3(grid), dim3(kBlock), 0, env->stream, a, q, sa);
