@@ -240,9 +240,10 @@ def batched_form(cg, seed: int, n_jobs: int, budget_gib: float, log) -> dict:
 
 
 def serving_form(cg, seed: int, rate: float, seconds: float, n_scenes: int, scene_scale: float, budget_gib: float,
-                 check: int, log) -> dict:
+                 check: int, log, texture_size: int = 2048) -> dict:
     """The serving form: n_scenes resident scenes (16-sphere generator at
-    `scene_scale`), Poisson arrivals at `rate` per second for `seconds`; each
+    `scene_scale`, each with four texture_size^2 RGBA8 material textures:
+    base colour, metallic-roughness, transmission, emission), Poisson arrivals at `rate` per second for `seconds`; each
     request: a scene, a camera on the orbit sphere, a resolution from
     {256x256, 640x360, 1280x720, 1920x1080} weighted 4:3:2:1, spp in
     {1, 4, 16}, depth 8.  Requests are admitted within the HBM budget and
@@ -251,7 +252,8 @@ def serving_form(cg, seed: int, rate: float, seconds: float, n_scenes: int, scen
     compared (relative MSE of the XYZ image)."""
     from crystalgraphics_amd import workloads
     t0 = time.perf_counter()
-    hosts = [cg.HostScene.generate("c1", seed=21 + i, params=[scene_scale]) for i in range(n_scenes)]
+    params = [scene_scale, 1, texture_size] if texture_size > 0 else [scene_scale]
+    hosts = [cg.HostScene.generate("c1", seed=21 + i, params=params) for i in range(n_scenes)]
     tris = workloads.scene_triangles(scene_scale) * n_scenes
     reqs = workloads.serve_requests(rate, seconds, n_scenes, seed=seed)
     srv = cg.Server(hosts, budget_bytes=int(budget_gib * (1 << 30)), max_trace_depth=8)
@@ -286,7 +288,8 @@ def serving_form(cg, seed: int, rate: float, seconds: float, n_scenes: int, scen
     pct = workloads.percentile
     lv = list(lat.values())
     return {
-        "workload": f"{n_scenes} resident scenes, {tris / 1e6:.1f}M tris total; Poisson {rate:.0f} req/s for "
+        "workload": f"{n_scenes} resident scenes, {tris / 1e6:.1f}M tris total, "
+                    f"{4 if texture_size > 0 else 0} x {texture_size}^2 RGBA8 textures per scene; Poisson {rate:.0f} req/s for "
                     f"{seconds:.0f} s; 256x256/640x360/1280x720/1920x1080 weighted 4:3:2:1, spp {{1,4,16}}, depth 8",
         "requests": len(reqs), "setup_s": round(setup, 1), "budget_gib": budget_gib,
         "latency_ms": {"p50": round(pct(lv, 50) * 1e3, 1), "p99": round(pct(lv, 99) * 1e3, 1),
@@ -317,6 +320,8 @@ def main() -> int:
     ap.add_argument("--serve-scene-scale", type=float, default=1.5390625,
                     help="sphere subdivision factor of each scene (1.539: ~4.9M tris per scene)")
     ap.add_argument("--serve-budget-gib", type=float, default=24.0)
+    ap.add_argument("--serve-texture-size", type=int, default=2048,
+                    help="side of the four RGBA8 material textures per serving scene (0: untextured)")
     ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--batch-budget-gib", type=float, default=16.0, help="HBM for the batch engine's slot pool")
     ap.add_argument("--probe-copy-gib", type=float, default=2.0,
@@ -420,7 +425,7 @@ def main() -> int:
     if not args.no_serve:
         serving = serving_form(cg, args.seed, args.serve_rate, args.serve_seconds, args.serve_scenes,
                                args.serve_scene_scale, args.serve_budget_gib, 4,
-                               lambda m: print(m, file=sys.stderr, flush=True))
+                               lambda m: print(m, file=sys.stderr, flush=True), args.serve_texture_size)
 
     base = None
     if not args.no_cpu:

@@ -75,17 +75,21 @@ class HostScene:
         return cls(h.value)
 
     @classmethod
-    def from_arrays(cls, vertices, indices, materials, prims, env=None, tag: int = 1) -> "HostScene":
+    def from_arrays(cls, vertices, indices, materials, prims, env=None, tag: int = 1, textures=None) -> "HostScene":
+        """textures: list of {"rgba8": (h, w, 4) uint8, "wrap_s", "wrap_t"}, indexed by the materials'
+        texture slots."""
         v = np.ascontiguousarray(vertices, VERTEX)
         i = np.ascontiguousarray(indices, np.uint32)
         m = np.ascontiguousarray(materials, MATERIAL)
         p = np.ascontiguousarray(prims, PRIMITIVE)
         e = None if env is None else np.ascontiguousarray(env, np.float32)
+        tex, keep = native.texture_structs(textures)
         h = C.c_void_p()
         rc = native.host().cgh_scene_create(
             v.ctypes.data, v.shape[0], i.ctypes.data, i.size, m.ctypes.data, m.shape[0], p.ctypes.data,
-            p.shape[0], None if e is None else e.ctypes.data, 0 if e is None else e.shape[1],
+            p.shape[0], tex, len(keep), None if e is None else e.ctypes.data, 0 if e is None else e.shape[1],
             0 if e is None else e.shape[0], tag, C.byref(h))
+        del keep
         _host_check(rc)
         return cls(h.value)
 
@@ -100,7 +104,13 @@ class HostScene:
             "env": None if not d.env else native.view_array(d.env, d.env_w * d.env_h * 4, np.dtype("<f4")).reshape(
                 d.env_h, d.env_w, 4),
             "camera": d.camera, "conf": d.conf, "tag": d.tag,
+            "textures": [],
         }
+        if d.texture_count:
+            views = (native.TextureC * d.texture_count).from_address(d.textures)
+            for t in views:
+                img = native.view_array(t.rgba8, t.width * t.height * 4, np.dtype("u1")).reshape(t.height, t.width, 4)
+                out["textures"].append({"rgba8": img, "wrap_s": t.wrap_s, "wrap_t": t.wrap_t})
         return out
 
     def scene_data(self, share_blas: bool = True, threads: int = 8) -> SceneData:

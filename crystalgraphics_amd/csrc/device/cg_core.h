@@ -179,6 +179,8 @@ struct DevScene {
   const float4* env;
   uint32_t env_w, env_h;
   uint32_t root;           // TLAS root ref
+  const uint32_t* tex;     // material texture array, RGBA8 texels (r in the low byte)
+  uint32_t tex_w, tex_h;
 };
 
 struct TravStats {
@@ -539,18 +541,88 @@ CG_HD float dispersed_ior(float nd, float disp, float l) {
   float inv_abbe = disp / 20.0f;
   return fmx(nd + (nd - 1.0f) * inv_abbe * (523655.0f / (l * l) - 1.5168f), 1.0f);
 }
-CG_HD Mat decode(const MatRec& m, float l, const SmitsBin& b) {
-  float refl = rgb_reflectance(ld3(m.base), b);
-  float real_ior = (m.metallic == 0.0f && m.transmission > 0.0f) ? dispersed_ior(m.ior, m.dispersion, l) : m.ior;
+
+// Bilinear lookup in one layer of the material texture array at level 0 with
+// repeat addressing: the sampler of src/wgpu/resource.cpp:589-606 as
+// textureSampleLevel applies it (material.wesl:170-171), in declared fp32
+// semantics.  The coordinate is wrapped into [0, 1] first (NaN and infinities
+// read as 0), texel centres sit at (i + 0.5) / size, texels read as k / 255,
+// and the four texels blend as two lerps across and one down, so a constant
+// region reads back exactly.  Sizes are at most 16384, so x0, y0 stay in
+// [-1, size - 1].
+struct Texel4 {
+  float r, g, b, a;
+};
+CG_HD float wrap01(float c) {
+  const float f = c - floorf(c);
+  return f >= 0.0f ? f : 0.0f;
+}
+CG_HD Texel4 unorm8x4(uint32_t t) {
+  return Texel4{static_cast<float>(t & 255u) / 255.0f, static_cast<float>((t >> 8) & 255u) / 255.0f,
+                static_cast<float>((t >> 16) & 255u) / 255.0f, static_cast<float>(t >> 24) / 255.0f};
+}
+CG_HD float lerp_decl(float p, float q, float w) { return p + w * (q - p); }
+CG_HD Texel4 tex_sample(const uint32_t* tex, uint32_t w, uint32_t h, uint32_t layer, f2 uv) {
+  const float x = wrap01(uv.x) * static_cast<float>(w) - 0.5f;
+  const float y = wrap01(uv.y) * static_cast<float>(h) - 0.5f;
+  const float fx = floorf(x), fy = floorf(y);
+  const float a = x - fx, b = y - fy;
+  const uint32_t x0 = fx < 0.0f ? w - 1u : umin(static_cast<uint32_t>(fx), w - 1u);
+  const uint32_t y0 = fy < 0.0f ? h - 1u : umin(static_cast<uint32_t>(fy), h - 1u);
+  const uint32_t x1 = x0 + 1u == w ? 0u : x0 + 1u;
+  const uint32_t y1 = y0 + 1u == h ? 0u : y0 + 1u;
+  const uint32_t* L = tex + static_cast<size_t>(layer) * w * h;
+  const Texel4 t00 = unorm8x4(L[static_cast<size_t>(y0) * w + x0]), t10 = unorm8x4(L[static_cast<size_t>(y0) * w + x1]);
+  const Texel4 t01 = unorm8x4(L[static_cast<size_t>(y1) * w + x0]), t11 = unorm8x4(L[static_cast<size_t>(y1) * w + x1]);
+  auto ch = [&](float p00, float p10, float p01, float p11) {
+    return lerp_decl(lerp_decl(p00, p10, a), lerp_decl(p01, p11, a), b);
+  };
+  return Texel4{ch(t00.r, t10.r, t01.r, t11.r), ch(t00.g, t10.g, t01.g, t11.g), ch(t00.b, t10.b, t01.b, t11.b),
+                ch(t00.a, t10.a, t01.a, t11.a)};
+}
+
+// Material decode at one wavelength with its texture lookups
+// (material.wesl:19-53 and the getters :181-262): factors times texture
+// samples, then clamps.  The metallic-roughness texture is sampled once for
+// both getters (they read the same texel).
+CG_HD Mat decode(const DevScene& S, const MatRec& m, float l, const SmitsBin& b, f2 uv0, f2 uv1) {
+  f3 base = ld3(m.base);
+  float rough = m.roughness, metal = m.metallic, trans = m.transmission;
+  f3 em = ld3(m.emission);
+  const uint32_t lb = m.tex_base_mr & 0xFFFFu, lmr = m.tex_base_mr >> 16;
+  const uint32_t lt = m.tex_trans_emit & 0xFFFFu, le = m.tex_trans_emit >> 16;
+  if (lb != CG_TEX_NONE) {
+    Texel4 t = tex_sample(S.tex, S.tex_w, S.tex_h, lb, (m.flags & CG_MAT_UV1_BASE) ? uv1 : uv0);
+    base = f3{base.x * t.r, base.y * t.g, base.z * t.b};
+  }
+  if (lmr != CG_TEX_NONE) {
+    Texel4 t = tex_sample(S.tex, S.tex_w, S.tex_h, lmr, (m.flags & CG_MAT_UV1_MR) ? uv1 : uv0);
+    rough = rough * t.g;
+    metal = metal * t.b;
+  }
+  if (lt != CG_TEX_NONE) {
+    Texel4 t = tex_sample(S.tex, S.tex_w, S.tex_h, lt, (m.flags & CG_MAT_UV1_TRANS) ? uv1 : uv0);
+    trans = trans * t.r;
+  }
+  if (le != CG_TEX_NONE) {
+    Texel4 t = tex_sample(S.tex, S.tex_w, S.tex_h, le, (m.flags & CG_MAT_UV1_EMIT) ? uv1 : uv0);
+    em = f3{em.x * t.r, em.y * t.g, em.z * t.b};
+  }
+  rough = clampf(rough, 0.0f, 1.0f);
+  metal = clampf(metal, 0.0f, 1.0f);
+  trans = clampf(trans, 0.0f, 1.0f);
+  em = f3{fmx(em.x * m.strength, 0.0f), fmx(em.y * m.strength, 0.0f), fmx(em.z * m.strength, 0.0f)};
+  float refl = rgb_reflectance(base, b);
+  float real_ior = (metal == 0.0f && trans > 0.0f) ? dispersed_ior(m.ior, m.dispersion, l) : m.ior;
   float np1 = real_ior + 1.0f;
   float im = CG_SQRT(clampf((4.0f * real_ior) / (1.0f - refl) - np1 * np1, 0.0f, 10.0f));
   Mat r;
-  r.diffuse = (1.0f - m.metallic) * (1.0f - m.transmission) * refl;
-  r.alpha = m.alpha;
-  r.ior = IOR{real_ior, m.metallic * im};
-  r.transmission = m.transmission;
-  r.volume = m.volume != 0u;
-  r.emission = rgb_reflectance(ld3(m.emission), b);
+  r.diffuse = (1.0f - metal) * (1.0f - trans) * refl;
+  r.alpha = rough * rough;
+  r.ior = IOR{real_ior, metal * im};
+  r.transmission = trans;
+  r.volume = (m.flags & CG_MAT_VOLUME) != 0u;
+  r.emission = rgb_reflectance(em, b);
   return r;
 }
 
@@ -983,7 +1055,7 @@ struct Frame5 {
 CG_HD void forward_pass(const DevScene& S, f3 d, float l, const SmitsBin& b, const Surface& s, int depth,
                         const TraceParams& P, uint32_t& rng, Frame5& x, const TStack& stk) {
   const MatRec& pm = S.mats[umin(s.material, S.mat_count - 1u)];
-  Mat m = decode(pm, l, b);
+  Mat m = decode(S, pm, l, b, s.uv0, s.uv1);
   Counts6 c = lobe_counts(m, S.alias_count != 0u, depth, P);
   x.tcount = 0u;
   x.ecount = 0u;

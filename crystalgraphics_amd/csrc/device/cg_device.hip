@@ -33,7 +33,7 @@ struct DevBuf {
 }  // namespace
 
 struct cg_scene {
-  DevBuf bvh, nodes, tris, insts, vidx, vattr, mats, emit, alias, emit_order, env;
+  DevBuf bvh, nodes, tris, insts, vidx, vattr, mats, emit, alias, emit_order, env, tex;
   DevScene view{};
   uint32_t stack_depth = 0;
   uint64_t bytes = 0;
@@ -530,6 +530,7 @@ void release_scene(cg_scene* s) {
   dev_free(s->alias);
   dev_free(s->emit_order);
   dev_free(s->env);
+  dev_free(s->tex);
 }
 
 // Resident grid of a kernel: blocks per CU at full occupancy x CUs, capped by
@@ -771,6 +772,11 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
   size_t env_bytes = static_cast<size_t>(sd->env_width) * sd->env_height * 4 * sizeof(float);
   total += env_bytes;
   up(dev_upload(env, s->env, sd->env_rgba32f, env_bytes));
+  const size_t tex_bytes = static_cast<size_t>(sd->tex_width) * sd->tex_height * 4 * sd->tex_layers;
+  if (tex_bytes) {
+    total += tex_bytes;
+    up(dev_upload(env, s->tex, sd->tex_rgba8, tex_bytes));
+  }
   if (rc == CG_OK && hipStreamSynchronize(env->stream) != hipSuccess)
     rc = fail(env, CG_ERR_HIP, "scene upload failed");
   if (rc != CG_OK) {
@@ -796,6 +802,9 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
   v.env = static_cast<const float4*>(s->env.p);
   v.env_w = sd->env_width;
   v.env_h = sd->env_height;
+  v.tex = static_cast<const uint32_t*>(s->tex.p);
+  v.tex_w = tex_bytes ? sd->tex_width : 0u;
+  v.tex_h = tex_bytes ? sd->tex_height : 0u;
   v.root = L.root;
   s->stack_depth = L.stack_depth;
   s->bytes = total;

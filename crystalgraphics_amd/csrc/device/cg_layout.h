@@ -12,7 +12,7 @@
 //                   in the merged world BLAS, the triangle's instance)
 //   InstRec   80 B  full column-major inverse transform, BLAS root ref, material
 //   VAttr     32 B  normal + both uv sets (read once, for the final hit)
-//   MatRec    64 B  the wavelength-independent part of a decoded material
+//   MatRec    64 B  material factors and texture layers
 //   EmitRec   64 B  emissive triangle, its unit normal, RGB and area pdf
 #ifndef CG_LAYOUT_H_
 #define CG_LAYOUT_H_
@@ -89,17 +89,28 @@ struct __attribute__((aligned(16))) VAttr {
   float _pad;
 };
 
+// Material factors as glTF gives them; the kernels multiply in the texture
+// samples and clamp (material.wesl:181-262).  Texture layers are 16-bit, the
+// value CG_TEX_NONE meaning no texture.
+#define CG_TEX_NONE 0xFFFFu
+#define CG_MAT_VOLUME 1u          // thickness_factor > 0
+#define CG_MAT_UV1_BASE 2u        // tex_coord == 1, per textured slot
+#define CG_MAT_UV1_MR 4u
+#define CG_MAT_UV1_TRANS 8u
+#define CG_MAT_UV1_EMIT 16u
 struct __attribute__((aligned(16))) MatRec {
-  float base[3];
-  float metallic;      // clamp(metallic_factor, 0, 1)
-  float roughness;     // clamp(roughness_factor, 0, 1)
-  float transmission;  // clamp(transmission_factor, 0, 1)
+  float base[3];       // base_color_factor rgb
+  float metallic;      // metallic_factor
+  float roughness;     // roughness_factor
+  float transmission;  // transmission_factor
   float ior;
   float dispersion;
-  float emission[3];   // max(emissive_factor * emissive_strength, 0)
-  uint32_t volume;     // thickness_factor > 0
-  float alpha;         // roughness * roughness
-  float _pad[3];
+  float emission[3];   // emissive_factor
+  float strength;      // emissive_strength
+  uint32_t flags;      // CG_MAT_*
+  uint32_t tex_base_mr;   // base colour layer | metallic-roughness layer << 16
+  uint32_t tex_trans_emit;  // transmission layer | emissive layer << 16
+  uint32_t _pad;
 };
 
 struct __attribute__((aligned(16))) EmitRec {

@@ -151,23 +151,38 @@ struct Layout {
 // Wavelength-independent records shared by both acceleration layouts:
 // materials, vertex attributes, emitters, alias table; plus input checks.
 inline int build_common(const cg_scene_data* sd, Layout& L, std::string& err) {
-  // Materials: textures are not implemented by the kernels.
+  // Materials.  A texture index past the array reads its last layer (WGSL
+  // clamps array indices); with no texture array every lookup reads white,
+  // which leaves the factor unchanged, so the slot is dropped.
+  if (sd->tex_layers > 0 && (!sd->tex_rgba8 || sd->tex_width == 0 || sd->tex_height == 0))
+    return set_error(err, CG_ERR_INVALID, "texture array has %u layers but no image data", sd->tex_layers);
+  if (sd->tex_layers > 0 && (sd->tex_width > 16384u || sd->tex_height > 16384u))
+    return set_error(err, CG_ERR_UNSUPPORTED, "texture array of %ux%u texels (max 16384x16384)", sd->tex_width,
+                     sd->tex_height);
+  if (sd->tex_layers >= CG_TEX_NONE)
+    return set_error(err, CG_ERR_UNSUPPORTED, "%u texture layers (max %u)", sd->tex_layers, CG_TEX_NONE - 1u);
+  auto layer = [&](const cg_tex_info& t) -> uint32_t {
+    if (t.index < 0 || sd->tex_layers == 0) return CG_TEX_NONE;
+    return std::min(static_cast<uint32_t>(t.index), sd->tex_layers - 1u);
+  };
   for (uint32_t i = 0; i < sd->material_count; ++i) {
     const cg_material& m = sd->materials[i];
-    for (int t : {CG_TEX_BASE_COLOR, CG_TEX_METALLIC_ROUGHNESS, CG_TEX_EMISSIVE, CG_TEX_TRANSMISSION})
-      if (m.tex[t].index >= 0)
-        return set_error(err, CG_ERR_UNSUPPORTED, "material %u uses texture slot %d: material textures are not supported",
-                    i, t);
     MatRec r{};
     for (int k = 0; k < 3; ++k) r.base[k] = m.base_color_factor[k];
-    r.metallic = fminf(fmaxf(m.metallic_factor, 0.0f), 1.0f);
-    r.roughness = fminf(fmaxf(m.roughness_factor, 0.0f), 1.0f);
-    r.transmission = fminf(fmaxf(m.transmission_factor, 0.0f), 1.0f);
+    r.metallic = m.metallic_factor;
+    r.roughness = m.roughness_factor;
+    r.transmission = m.transmission_factor;
     r.ior = m.ior;
     r.dispersion = m.dispersion;
-    for (int k = 0; k < 3; ++k) r.emission[k] = fmaxf(m.emissive_factor[k] * m.emissive_strength, 0.0f);
-    r.volume = m.thickness_factor > 0.0f ? 1u : 0u;
-    r.alpha = r.roughness * r.roughness;
+    for (int k = 0; k < 3; ++k) r.emission[k] = m.emissive_factor[k];
+    r.strength = m.emissive_strength;
+    r.flags = m.thickness_factor > 0.0f ? CG_MAT_VOLUME : 0u;
+    if (m.tex[CG_TEX_BASE_COLOR].tex_coord == 1u) r.flags |= CG_MAT_UV1_BASE;
+    if (m.tex[CG_TEX_METALLIC_ROUGHNESS].tex_coord == 1u) r.flags |= CG_MAT_UV1_MR;
+    if (m.tex[CG_TEX_TRANSMISSION].tex_coord == 1u) r.flags |= CG_MAT_UV1_TRANS;
+    if (m.tex[CG_TEX_EMISSIVE].tex_coord == 1u) r.flags |= CG_MAT_UV1_EMIT;
+    r.tex_base_mr = layer(m.tex[CG_TEX_BASE_COLOR]) | layer(m.tex[CG_TEX_METALLIC_ROUGHNESS]) << 16;
+    r.tex_trans_emit = layer(m.tex[CG_TEX_TRANSMISSION]) | layer(m.tex[CG_TEX_EMISSIVE]) << 16;
     L.mats.push_back(r);
   }
   if (L.mats.empty()) return set_error(err, CG_ERR_INVALID, "scene has no materials");

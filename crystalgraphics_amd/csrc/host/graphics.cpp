@@ -207,9 +207,23 @@ int cgh_scene_generate(const char* name, uint64_t seed, const double* params, in
 }
 
 int cgh_scene_create(const cg_vertex* v, uint32_t nv, const uint32_t* idx, uint32_t ni, const cg_material* m,
-                     uint32_t nm, const cg_primitive* p, uint32_t np, const float* env, uint32_t ew, uint32_t eh,
-                     uint64_t tag, cgh_scene** out) {
+                     uint32_t nm, const cg_primitive* p, uint32_t np, const cg_texture* tex, uint32_t ntex,
+                     const float* env, uint32_t ew, uint32_t eh, uint64_t tag, cgh_scene** out) {
+  *out = nullptr;
   auto impl = std::make_unique<Scene::Impl>();
+  for (uint32_t i = 0; i < ntex; ++i) {
+    if (!tex[i].rgba8 || tex[i].width == 0 || tex[i].height == 0) {
+      t_err = "texture " + std::to_string(i) + " has no image data";
+      return CG_ERR_INVALID;
+    }
+    HostTexture t;
+    t.width = tex[i].width;
+    t.height = tex[i].height;
+    t.wrap_s = tex[i].wrap_s;
+    t.wrap_t = tex[i].wrap_t;
+    t.rgba8.assign(tex[i].rgba8, tex[i].rgba8 + static_cast<size_t>(t.width) * t.height * 4);
+    impl->textures.push_back(std::move(t));
+  }
   impl->vertices.assign(v, v + nv);
   impl->indices.assign(idx, idx + ni);
   impl->materials.assign(m, m + nm);
@@ -235,6 +249,8 @@ typedef struct cgh_scene_desc {
   uint32_t material_count;
   const cg_primitive* prims;
   uint32_t prim_count;
+  const cg_texture* textures;  // valid while the scene lives
+  uint32_t texture_count;
   const float* env;
   uint32_t env_w, env_h;
   cg_camera camera;
@@ -244,6 +260,9 @@ typedef struct cgh_scene_desc {
 
 int cgh_scene_describe(const cgh_scene* s, cgh_scene_desc* d) {
   const auto* impl = &impl_of(s);
+  impl->texture_views.clear();
+  for (const auto& t : impl->textures)
+    impl->texture_views.push_back(cg_texture{t.rgba8.data(), t.width, t.height, t.wrap_s, t.wrap_t});
   d->vertices = impl->vertices.data();
   d->vertex_count = static_cast<uint32_t>(impl->vertices.size());
   d->indices = impl->indices.data();
@@ -252,6 +271,8 @@ int cgh_scene_describe(const cgh_scene* s, cgh_scene_desc* d) {
   d->material_count = static_cast<uint32_t>(impl->materials.size());
   d->prims = impl->prims.data();
   d->prim_count = static_cast<uint32_t>(impl->prims.size());
+  d->textures = impl->texture_views.empty() ? nullptr : impl->texture_views.data();
+  d->texture_count = static_cast<uint32_t>(impl->texture_views.size());
   d->env = impl->env_rgba32f.empty() ? nullptr : impl->env_rgba32f.data();
   d->env_w = impl->env_w;
   d->env_h = impl->env_h;

@@ -17,6 +17,10 @@
 //             lengths U(0.001, 0.01).
 //   "quad"    one diffuse quad under the sky (a hand-checkable scene).
 //   "spheres" params[0] spheres (default 8), for small parity cases.
+// "c1" and "spheres" take params[1] > 0 to add four procedural material
+// textures (base colour, metallic-roughness, transmission, emission) of side
+// params[2] (c1: 2048 each; spheres: 64, with layers of different sizes and
+// wrap modes) and reference them from the materials.
 // Materials are drawn uniformly from {diffuse albedo U(0.2, 0.9), rough metal
 // GGX alpha U(0.05, 0.5), dispersive glass IOR 1.5}; glTF's dispersion for
 // Cauchy B = 0.004 um^2 at n_d = 1.5 is 20 B / ((n_d - 1) 523655 nm^2) = 0.30554.
@@ -141,6 +145,7 @@ struct Builder {
     s.vertices.clear();
     s.indices.clear();
     s.prims.clear();
+    s.textures.clear();
     s.materials.assign(1, DefaultMaterial());  // default material at index 0 (scene.cpp:869-872)
   }
   uint32_t material(const cg_material& m) {
@@ -162,8 +167,8 @@ struct Builder {
       D3 d = dnorm(pos[i]);
       v.uv0[0] = static_cast<float>(0.5 + std::atan2(d.y, d.x) / 6.283185307179586);
       v.uv0[1] = static_cast<float>(std::acos(std::max(-1.0, std::min(1.0, d.z))) / 3.141592653589793);
-      v.uv1[0] = v.uv0[0];
-      v.uv1[1] = v.uv0[1];
+      v.uv1[0] = 2.0f * v.uv0[0] - 0.5f;
+      v.uv1[1] = 3.0f * v.uv0[1] + 0.25f;
       s.vertices.push_back(v);
     }
     for (uint32_t t : tri) s.indices.push_back(vo + t);
@@ -363,6 +368,60 @@ void set_conf(Scene::Impl& s, uint32_t w, uint32_t h, uint32_t spp, uint32_t dep
   s.conf = cg_conf{1920, 1080, w, h, spp, depth, 3, 1, 1, 1};
 }
 
+// Procedural material textures: an 8x8 checker of per-tile values with
+// per-texel hash noise on top (so that neighbouring texels differ and the
+// bilinear weights matter).  kind 0 base colour, 1 metallic-roughness (G
+// roughness, B metallic mask), 2 transmission (R), 3 emission.
+uint32_t mix32(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xFF51AFD7ED558CCDull;
+  x ^= x >> 33;
+  x *= 0xC4CEB9FE1A85EC53ull;
+  x ^= x >> 33;
+  return static_cast<uint32_t>(x);
+}
+HostTexture make_texture(int kind, uint32_t w, uint32_t h, uint64_t seed, int32_t wrap_s, int32_t wrap_t) {
+  HostTexture t;
+  t.width = w;
+  t.height = h;
+  t.wrap_s = wrap_s;
+  t.wrap_t = wrap_t;
+  t.rgba8.resize(static_cast<size_t>(w) * h * 4);
+  const uint64_t key = seed * 0x9E3779B97F4A7C15ull + static_cast<uint64_t>(kind) * 0x632BE59BD9B4E019ull;
+  for (uint32_t y = 0; y < h; ++y)
+    for (uint32_t x = 0; x < w; ++x) {
+      const uint32_t tile = mix32(key ^ ((static_cast<uint64_t>(x * 8u / w) << 8) | (y * 8u / h)));
+      const uint32_t n = mix32(key + (static_cast<uint64_t>(y) << 32 | x));
+      auto ch = [&](int sh, int lo, int span) { return static_cast<uint8_t>(lo + ((tile >> sh) & 0xFFu) * span / 255 + (n >> sh) % 24u); };
+      uint8_t* px = &t.rgba8[(static_cast<size_t>(y) * w + x) * 4];
+      switch (kind) {
+        case 0: px[0] = ch(0, 40, 190); px[1] = ch(8, 40, 190); px[2] = ch(16, 40, 190); px[3] = 255; break;
+        case 1: px[0] = 255; px[1] = ch(0, 50, 180); px[2] = (tile & 0x100u) ? 255 : static_cast<uint8_t>(n % 16u); px[3] = 255; break;
+        case 2: px[0] = ch(0, 120, 110); px[1] = px[2] = 0; px[3] = 255; break;
+        default: px[0] = ch(0, 60, 170); px[1] = ch(8, 60, 170); px[2] = ch(16, 60, 170); px[3] = 255; break;
+      }
+    }
+  return t;
+}
+cg_tex_info tex_ref(int index, uint32_t tex_coord) { return cg_tex_info{index, tex_coord, 1.0f, 1.0f}; }
+// Texture the scene's materials with layers 0-3: base colour on diffuse and
+// metal, metallic-roughness (through uv1) on metal, transmission on glass,
+// emission (through uv1) on emitters.
+void texture_materials(Scene::Impl& s) {
+  for (size_t i = 1; i < s.materials.size(); ++i) {
+    cg_material& m = s.materials[i];
+    const bool emitter = m.emissive_factor[0] > 0.0f || m.emissive_factor[1] > 0.0f || m.emissive_factor[2] > 0.0f;
+    if (emitter) {
+      m.tex[CG_TEX_EMISSIVE] = tex_ref(3, 1);
+    } else if (m.transmission_factor > 0.0f) {
+      m.tex[CG_TEX_TRANSMISSION] = tex_ref(2, 0);
+    } else {
+      m.tex[CG_TEX_BASE_COLOR] = tex_ref(0, 0);
+      if (m.metallic_factor > 0.0f) m.tex[CG_TEX_METALLIC_ROUGHNESS] = tex_ref(1, 1);
+    }
+  }
+}
+
 }  // namespace
 
 cg_material DefaultMaterial() {  // src/scene.cpp:659-706
@@ -447,6 +506,11 @@ std::string GenerateScene(const std::string& name, uint64_t seed, const double* 
       m.metallic_factor = 0.0f;
       B.prim(B.mesh(tp, tn, {0, 1, 2}), B.material(m), identity());
     }
+    if (param(1, 0.0) > 0.0) {
+      const uint32_t ts = static_cast<uint32_t>(param(2, 2048.0));
+      for (int k = 0; k < 4; ++k) s.textures.push_back(make_texture(k, ts, ts, seed, CG_WRAP_REPEAT, CG_WRAP_REPEAT));
+      texture_materials(s);
+    }
     set_camera(s, D3{0.0, -15.0, 5.0}, D3{0.0, 0.0, 0.0}, 1280, 720);
     set_conf(s, 1280, 720, 16, 8);
   } else if (name == "c2") {
@@ -512,12 +576,27 @@ std::string GenerateScene(const std::string& name, uint64_t seed, const double* 
     light.metallic_factor = 0.0f;
     quad(0.5, 2.5, true, pos, nrm, tri);
     B.prim(B.mesh(pos, nrm, tri), B.material(light), identity());
+    if (param(1, 0.0) > 0.0) {  // layers of different sizes and wrap modes
+      const uint32_t ts = static_cast<uint32_t>(param(2, 64.0));
+      s.textures.push_back(make_texture(0, ts, ts, seed, CG_WRAP_REPEAT, CG_WRAP_REPEAT));
+      s.textures.push_back(make_texture(1, std::max(1u, ts / 2), ts + 3, seed, CG_WRAP_CLAMP_TO_EDGE, CG_WRAP_REPEAT));
+      s.textures.push_back(make_texture(2, 7, 5, seed, CG_WRAP_MIRRORED_REPEAT, CG_WRAP_MIRRORED_REPEAT));
+      s.textures.push_back(make_texture(3, ts, std::max(1u, ts / 4), seed, CG_WRAP_MIRRORED_REPEAT, CG_WRAP_CLAMP_TO_EDGE));
+      texture_materials(s);
+      s.materials[1].tex[CG_TEX_BASE_COLOR] = tex_ref(0, 1);  // the ground, through uv1
+    }
     set_camera(s, D3{0.0, -4.5, 2.0}, D3{0.0, 0.0, 0.5}, 96, 64);
     set_conf(s, 96, 64, 4, 4);
   } else {
     return "unknown procedural scene '" + name + "'";
   }
-  s.tag = Pcg32::splitmix(seed) ^ std::hash<std::string>{}(name);
+  uint64_t tag_seed = seed;
+  for (int i = 0; params && i < nparams; ++i) {
+    uint64_t bits;
+    std::memcpy(&bits, &params[i], sizeof bits);
+    tag_seed = tag_seed * 0x100000001B3ull ^ bits;
+  }
+  s.tag = Pcg32::splitmix(tag_seed) ^ std::hash<std::string>{}(name);
   return "";
 }
 

@@ -28,6 +28,38 @@ struct V3 {
 inline V3 ld(const float* p) { return V3{p[0], p[1], p[2]}; }
 inline V3 vmin(V3 a, V3 b) { return V3{std::fmin(a.x, b.x), std::fmin(a.y, b.y), std::fmin(a.z, b.z)}; }
 inline V3 vmax(V3 a, V3 b) { return V3{std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z)}; }
+
+// Emissive texture factor at a triangle's centroid uv
+// (src/pathtracing/emissive_buffer.h:135-212): nearest texel of the decoded
+// image under the texture's own wrap modes.  A NaN coordinate reads texel 0.
+float wrap_coord(float c, int32_t wrap) {  // emissive_buffer.h:191-204
+  float w;
+  if (wrap == CG_WRAP_CLAMP_TO_EDGE) {
+    w = c;
+  } else if (wrap == CG_WRAP_MIRRORED_REPEAT) {
+    const float fl = std::floor(c);
+    w = c - fl;
+    if (std::fmod(fl, 2.0f) != 0.0f) w = 1.0f - w;
+  } else {
+    w = c - std::floor(c);
+  }
+  if (!(w >= 0.0f)) w = 0.0f;  // NaN
+  return std::min(w, 0.999999f);
+}
+V3 emissive_texture_factor(const Scene::Impl& sc, const cg_material& m, const cg_vertex& a, const cg_vertex& b,
+                           const cg_vertex& c) {
+  const cg_tex_info& ti = m.tex[CG_TEX_EMISSIVE];
+  if (ti.index < 0 || static_cast<size_t>(ti.index) >= sc.textures.size()) return V3{1.0f, 1.0f, 1.0f};
+  const HostTexture& t = sc.textures[static_cast<size_t>(ti.index)];
+  const float* ua = ti.tex_coord == 1 ? a.uv1 : a.uv0;
+  const float* ub = ti.tex_coord == 1 ? b.uv1 : b.uv0;
+  const float* uc = ti.tex_coord == 1 ? c.uv1 : c.uv0;
+  const float u = ((ua[0] + ub[0]) + uc[0]) / 3.0f, v = ((ua[1] + ub[1]) + uc[1]) / 3.0f;
+  const uint32_t x = std::min(t.width - 1, static_cast<uint32_t>(wrap_coord(u, t.wrap_s) * static_cast<float>(t.width)));
+  const uint32_t y = std::min(t.height - 1, static_cast<uint32_t>(wrap_coord(v, t.wrap_t) * static_cast<float>(t.height)));
+  const uint8_t* px = &t.rgba8[(static_cast<size_t>(y) * t.width + x) * 4];
+  return V3{static_cast<float>(px[0]) / 255.0f, static_cast<float>(px[1]) / 255.0f, static_cast<float>(px[2]) / 255.0f};
+}
 inline V3 add(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
 inline V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
 inline V3 mul(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
@@ -209,7 +241,24 @@ cg_scene_data SceneData::View() const {
   d.env_rgba32f = env.data();
   d.env_width = env_w;
   d.env_height = env_h;
+  d.tex_rgba8 = tex.empty() ? nullptr : tex.data();
+  d.tex_width = tex_w;
+  d.tex_height = tex_h;
+  d.tex_layers = tex_layers;
   return d;
+}
+
+std::vector<uint8_t> ResizeRgba8Nearest(const HostTexture& t, uint32_t w, uint32_t h) {
+  if (t.width == w && t.height == h) return t.rgba8;
+  std::vector<uint8_t> out(static_cast<size_t>(w) * h * 4);
+  for (uint32_t y = 0; y < h; ++y) {
+    const uint32_t sy = std::min(t.height - 1, static_cast<uint32_t>(static_cast<uint64_t>(y) * t.height / h));
+    for (uint32_t x = 0; x < w; ++x) {
+      const uint32_t sx = std::min(t.width - 1, static_cast<uint32_t>(static_cast<uint64_t>(x) * t.width / w));
+      std::memcpy(&out[(static_cast<size_t>(y) * w + x) * 4], &t.rgba8[(static_cast<size_t>(sy) * t.width + sx) * 4], 4);
+    }
+  }
+  return out;
 }
 
 SceneData BuildSceneData(const Scene::Impl& sc, bool share_blas, int threads) {
@@ -357,7 +406,8 @@ SceneData BuildSceneData(const Scene::Impl& sc, bool share_blas, int threads) {
       const V3 cr = cross(sub(p1, p0), sub(p2, p0));
       const float area = std::sqrt(dot(cr, cr)) * 0.5f;
       if (area <= 0.0f) continue;
-      const V3 em{base.x * 1.0f, base.y * 1.0f, base.z * 1.0f};  // no emissive texture
+      const V3 f = emissive_texture_factor(sc, m, verts[idx[o]], verts[idx[o + 1]], verts[idx[o + 2]]);
+      const V3 em{base.x * f.x, base.y * f.y, base.z * f.z};
       const float w = lum(em);
       if (w <= 0.0f) continue;
       cg_emissive_prim e{};
@@ -377,6 +427,22 @@ SceneData BuildSceneData(const Scene::Impl& sc, bool share_blas, int threads) {
   weights.reserve(sd.emissive.size());
   for (const auto& e : sd.emissive) weights.push_back(static_cast<double>(e.weight));
   sd.alias = BuildAliasTable(weights);
+
+  // Material texture array (src/wgpu/resource.cpp:471-546).
+  if (!sc.textures.empty()) {
+    sd.tex_w = sd.tex_h = 1;
+    for (const auto& t : sc.textures) {
+      sd.tex_w = std::max(sd.tex_w, t.width);
+      sd.tex_h = std::max(sd.tex_h, t.height);
+    }
+    sd.tex_layers = static_cast<uint32_t>(sc.textures.size());
+    const size_t layer = static_cast<size_t>(sd.tex_w) * sd.tex_h * 4;
+    sd.tex.resize(layer * sd.tex_layers);
+    for (uint32_t l = 0; l < sd.tex_layers; ++l) {
+      std::vector<uint8_t> r = ResizeRgba8Nearest(sc.textures[l], sd.tex_w, sd.tex_h);
+      std::memcpy(&sd.tex[layer * l], r.data(), layer);
+    }
+  }
 
   // Environment (scene_data.h:57-98).
   if (!sc.env_rgba32f.empty() && sc.env_w && sc.env_h) {

@@ -86,14 +86,42 @@ class SceneDataC(C.Structure):
                 ("materials", C.c_void_p), ("material_count", C.c_uint32),
                 ("emissive", C.c_void_p), ("emissive_count", C.c_uint32),
                 ("alias", C.c_void_p), ("alias_count", C.c_uint32),
-                ("env_rgba32f", C.c_void_p), ("env_width", C.c_uint32), ("env_height", C.c_uint32)]
+                ("env_rgba32f", C.c_void_p), ("env_width", C.c_uint32), ("env_height", C.c_uint32),
+                ("tex_rgba8", C.c_void_p), ("tex_width", C.c_uint32), ("tex_height", C.c_uint32),
+                ("tex_layers", C.c_uint32)]
+
+
+class TextureC(C.Structure):
+    """cg_texture"""
+    _fields_ = [("rgba8", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("wrap_s", C.c_int32),
+                ("wrap_t", C.c_int32)]
+
+
+WRAP_REPEAT, WRAP_CLAMP_TO_EDGE, WRAP_MIRRORED_REPEAT = 10497, 33071, 33648
+
+
+def texture_structs(textures) -> tuple[C.Array | None, list]:
+    """A cg_texture array over a list of {"rgba8": (h, w, 4) uint8, "wrap_s", "wrap_t"}; returns it and the
+    arrays to keep alive."""
+    if not textures:
+        return None, []
+    keep = []
+    arr = (TextureC * len(textures))()
+    for k, t in enumerate(textures):
+        img = np.ascontiguousarray(t["rgba8"], np.uint8)
+        assert img.ndim == 3 and img.shape[2] == 4, "texture images are (height, width, 4) uint8"
+        keep.append(img)
+        arr[k] = TextureC(img.ctypes.data, img.shape[1], img.shape[0], int(t.get("wrap_s", WRAP_REPEAT)),
+                          int(t.get("wrap_t", WRAP_REPEAT)))
+    return arr, keep
 
 
 class SceneDescC(C.Structure):
     """cgh_scene_desc"""
     _fields_ = [("vertices", C.c_void_p), ("vertex_count", C.c_uint32), ("indices", C.c_void_p),
                 ("index_count", C.c_uint32), ("materials", C.c_void_p), ("material_count", C.c_uint32),
-                ("prims", C.c_void_p), ("prim_count", C.c_uint32), ("env", C.c_void_p),
+                ("prims", C.c_void_p), ("prim_count", C.c_uint32), ("textures", C.c_void_p),
+                ("texture_count", C.c_uint32), ("env", C.c_void_p),
                 ("env_w", C.c_uint32), ("env_h", C.c_uint32), ("camera", Camera), ("conf", Conf),
                 ("tag", C.c_uint64)]
 
@@ -151,6 +179,8 @@ def scene_data_arrays(sd: SceneDataC) -> dict:
         out[ptr] = view_array(getattr(sd, ptr), getattr(sd, cnt), dt)
     out["env"] = view_array(sd.env_rgba32f, sd.env_width * sd.env_height * 4, np.dtype("<f4")).reshape(
         sd.env_height, sd.env_width, 4)
+    out["tex"] = view_array(sd.tex_rgba8, sd.tex_layers * sd.tex_height * sd.tex_width * 4, np.dtype("u1")).reshape(
+        sd.tex_layers, sd.tex_height, sd.tex_width, 4)
     return out
 
 
@@ -168,6 +198,12 @@ def make_scene_data(arrays: dict) -> tuple[SceneDataC, list]:
     keep.append(env)
     sd.env_rgba32f = env.ctypes.data
     sd.env_height, sd.env_width = env.shape[0], env.shape[1]
+    tex = arrays.get("tex")
+    if tex is not None and tex.size:
+        tex = np.ascontiguousarray(tex, dtype=np.uint8)
+        keep.append(tex)
+        sd.tex_rgba8 = tex.ctypes.data
+        sd.tex_layers, sd.tex_height, sd.tex_width = tex.shape[0], tex.shape[1], tex.shape[2]
     return sd, keep
 
 
@@ -248,7 +284,8 @@ def host() -> C.CDLL:
         h = _load("libcrystal_graphics.so")
         _sig(h, "cgh_last_error", C.c_char_p)
         _sig(h, "cgh_scene_generate", I32, C.c_char_p, C.c_uint64, P, I32, C.POINTER(P))
-        _sig(h, "cgh_scene_create", I32, P, U32, P, U32, P, U32, P, U32, P, U32, U32, C.c_uint64, C.POINTER(P))
+        _sig(h, "cgh_scene_create", I32, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, U32, C.c_uint64,
+             C.POINTER(P))
         _sig(h, "cgh_scene_free", None, P)
         _sig(h, "cgh_scene_describe", I32, P, C.POINTER(SceneDescC))
         _sig(h, "cgh_rays_generate", I32, P, C.c_uint64, U32, P)

@@ -83,11 +83,13 @@ enum { CG_ACCEL_REFERENCE = 0, CG_ACCEL_REBUILT = 1 };
 int cg_env_set_accel(cg_env* env, int mode);
 int cg_stats_reset(cg_env* env);
 
-/* Upload a flattened scene and build the device layout (pair nodes, triangle
- * records, instance records, material and emitter records, wavelength and
- * alias tables).  Replaces WriteScene + UpdateComputeBindGroup2
- * (src/wgpu/resource.cpp:338-414, src/wgpu/compute.cpp:332-372).
- * Materials with textures give CG_ERR_UNSUPPORTED. */
+/* Upload a flattened scene and build the device layout (quantised 8-wide BVH
+ * nodes, triangle records, instance records, material and emitter records,
+ * alias table, environment and material texture arrays).  Replaces
+ * WriteScene + WriteMaterialTextures + UpdateComputeBindGroup2
+ * (src/wgpu/resource.cpp:338-414,471-546, src/wgpu/compute.cpp:332-372).
+ * Texture arrays larger than 16384 x 16384 per layer, or of 65535 layers or
+ * more, give CG_ERR_UNSUPPORTED. */
 int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out);
 int cg_scene_release(cg_env* env, cg_scene* scene);
 /* Host-only check of the device layout cg_scene_upload would build (no GPU

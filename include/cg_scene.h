@@ -11,6 +11,7 @@
  * Reference layouts mirrored here:
  *   cg_vertex        src/vertex.h:12-17                     48 B
  *   cg_tex_info      src/material.h:23-28                   16 B
+ *   cg_texture       src/texture.h:12-22 after DecodeImageRgba8 (scene input)
  *   cg_material      src/material.h:30-87                  432 B
  *   cg_primitive     src/scene_impl.h:24-30 + AbsTrans()    (scene input)
  *   cg_tlas_node     src/pathtracing/tlas.h:17-30           32 B
@@ -48,6 +49,22 @@ typedef struct cg_tex_info {
   float scale;
   float strength;
 } cg_tex_info;
+
+/* A decoded material texture (the reference's TextureData after
+ * DecodeImageRgba8, src/texture.h:12-22, src/image_decode.h): RGBA8 texels,
+ * rows top to bottom, and the glTF sampler's wrap modes.  Only the emissive
+ * buffer's centroid lookup honours wrap_s / wrap_t
+ * (src/pathtracing/emissive_buffer.h:170-204); the device sampler always
+ * repeats (src/wgpu/resource.cpp:589-606). */
+typedef struct cg_texture {
+  const uint8_t* rgba8;  /* width * height * 4 bytes */
+  uint32_t width, height;
+  int32_t wrap_s, wrap_t;
+} cg_texture;
+
+#define CG_WRAP_REPEAT 10497
+#define CG_WRAP_CLAMP_TO_EDGE 33071
+#define CG_WRAP_MIRRORED_REPEAT 33648
 
 enum {
   CG_TEX_BASE_COLOR = 0,
@@ -210,7 +227,13 @@ typedef struct cg_hit {
 /* The flattened scene the reference's SceneData hands its device backend
  * (src/pathtracing/scene_data.h:23-54, uploaded by src/wgpu/resource.cpp:
  * 338-414), plus `tri_ids`: for each BVH-ordered triangle slot, its index in
- * its primitive's source index buffer (the reference keeps no triangle id). */
+ * its primitive's source index buffer (the reference keeps no triangle id).
+ *
+ * The material texture array is what WriteMaterialTextures uploads
+ * (src/wgpu/resource.cpp:471-546): tex_layers images (layer = glTF texture
+ * index), each resized nearest to the largest width and height, RGBA8 texels
+ * row-major, top row first.  tex_layers == 0 stands for the reference's 1x1
+ * white layer: every texture lookup then reads 1.0. */
 typedef struct cg_scene_data {
   const cg_tlas_node* tlas_nodes;   uint32_t tlas_node_count;
   const cg_instance* instances;     uint32_t instance_count;
@@ -222,6 +245,7 @@ typedef struct cg_scene_data {
   const cg_emissive_prim* emissive; uint32_t emissive_count;
   const cg_alias_entry* alias;      uint32_t alias_count;
   const float* env_rgba32f;         uint32_t env_width, env_height;
+  const uint8_t* tex_rgba8;         uint32_t tex_width, tex_height, tex_layers;
 } cg_scene_data;
 
 #ifdef __cplusplus

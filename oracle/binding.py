@@ -30,7 +30,7 @@ def lib() -> C.CDLL:
             raise ImportError(f"{_PATH} is not built (make -C oracle)")
         L = C.CDLL(str(_PATH))
         sig = [
-            ("orc_scene_build", P, [P, U32, P, U32, P, U32, P, U32, P, U32, U32]),
+            ("orc_scene_build", P, [P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, U32]),
             ("orc_scene_wrap", P, [C.POINTER(SceneDataC)]),
             ("orc_scene_free", None, [P]),
             ("orc_scene_get", None, [P, C.POINTER(SceneDataC)]),
@@ -38,6 +38,8 @@ def lib() -> C.CDLL:
             ("orc_render", C.c_int, [P, C.POINTER(Conf), C.POINTER(Camera), U32, P, P, P, P, U32, U32, C.c_int,
                                      C.c_int]),
             ("orc_sincos", None, [F, FP, FP]),
+            ("orc_texture_sample", None, [P, U32, U32, U32, C.c_int32, F, F, FP]),
+            ("orc_emissive_texel", None, [C.POINTER(native.TextureC), F, F, FP]),
             ("orc_rng_step", U32, [U32]),
             ("orc_rand_float", F, [C.POINTER(U32)]),
             ("orc_erf", F, [F]),
@@ -66,6 +68,23 @@ def lib() -> C.CDLL:
     return _lib
 
 
+def texture_sample(layers: np.ndarray, layer: int, u: float, v: float) -> np.ndarray:
+    """The oracle's material texture lookup; layers is (L, H, W, 4) uint8 (L may be 0)."""
+    a = np.ascontiguousarray(layers, np.uint8)
+    out = (C.c_float * 4)()
+    lib().orc_texture_sample(a.ctypes.data if a.size else None, a.shape[2], a.shape[1], a.shape[0], layer, u, v, out)
+    return np.array(out[:], np.float32)
+
+
+def emissive_texel(image: np.ndarray, wrap_s: int, wrap_t: int, u: float, v: float) -> np.ndarray:
+    """The emissive buffer's nearest lookup in a (H, W, 4) uint8 image."""
+    img = np.ascontiguousarray(image, np.uint8)
+    t = native.TextureC(img.ctypes.data, img.shape[1], img.shape[0], wrap_s, wrap_t)
+    out = (C.c_float * 3)()
+    lib().orc_emissive_texel(C.byref(t), u, v, out)
+    return np.array(out[:], np.float32)
+
+
 def fvec(values) -> C.Array:
     a = (C.c_float * len(values))(*[float(v) for v in values])
     return a
@@ -86,9 +105,11 @@ class OracleScene:
         p = np.ascontiguousarray(desc["prims"], PRIMITIVE)
         e = desc.get("env")
         e = None if e is None else np.ascontiguousarray(e, np.float32)
+        tex, keep = native.texture_structs(desc.get("textures"))
         h = lib().orc_scene_build(v.ctypes.data, v.shape[0], i.ctypes.data, i.size, m.ctypes.data, m.shape[0],
-                                  p.ctypes.data, p.shape[0], None if e is None else e.ctypes.data,
+                                  p.ctypes.data, p.shape[0], tex, len(keep), None if e is None else e.ctypes.data,
                                   0 if e is None else e.shape[1], 0 if e is None else e.shape[0])
+        del keep
         return cls(h)
 
     @classmethod

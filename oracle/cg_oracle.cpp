@@ -287,18 +287,93 @@ float imag_ior(float n, float c) {  // material.wesl:99-110
   float np1 = n + 1.0f;
   return std::sqrt(clampf((4.0f * n) / (1.0f - c) - np1 * np1, 0.0f, 10.0f));
 }
-Mat decode_material(const cg_material& pm, float l) {  // material.wesl:19-53
-  v3 base = mk(pm.base_color_factor[0], pm.base_color_factor[1], pm.base_color_factor[2]);
-  float refl = rgb_to_reflectance(base, l);
-  float rough = clampf(pm.roughness_factor, 0.0f, 1.0f);
-  float metal = clampf(pm.metallic_factor, 0.0f, 1.0f);
-  float trans = clampf(pm.transmission_factor, 0.0f, 1.0f);
+// The material texture array as WriteMaterialTextures uploads it
+// (src/wgpu/resource.cpp:471-546); layers == 0 is the 1x1 white default.
+struct TexArr {
+  const uint8_t* rgba8 = nullptr;
+  uint32_t w = 0, h = 0, layers = 0;
+};
+// textureSampleLevel(material_textures, sampler, uv, layer, 0) with the
+// linear / repeat sampler of resource.cpp:589-606 (material.wesl:181-262), in
+// the declared fp32 semantics:
+//   s' = s - floor(s), NaN -> 0            repeat, in normalised coordinates
+//   u = s' * W - 0.5, i0 = floor(u), alpha = u - i0   (texel centres at i + 1/2)
+//   i0 -> i0 mod W, i1 = (i0 + 1) mod W   (same in t with H, beta)
+//   texel channel = k / 255               (rgba8unorm)
+//   value = lerp(lerp(T00, T10, alpha), lerp(T01, T11, alpha), beta),
+//   lerp(p, q, w) = p + w * (q - p)
+// A layer index past the array reads the last layer (WGSL clamps it).
+struct Rgba {
+  float c[4];
+};
+Rgba sample_texture(const TexArr& T, int32_t index, f2 uv) {
+  Rgba out{{1.0f, 1.0f, 1.0f, 1.0f}};
+  if (T.layers == 0) return out;  // the white 1x1 default layer
+  const uint32_t layer = std::min(static_cast<uint32_t>(index), T.layers - 1u);
+  const uint32_t dims[2] = {T.w, T.h};
+  const float st[2] = {uv.x, uv.y};
+  uint32_t lo[2], hi[2];
+  float frac[2];
+  for (int a = 0; a < 2; ++a) {
+    float r = st[a] - std::floor(st[a]);
+    if (std::isnan(r)) r = 0.0f;
+    const float u = r * static_cast<float>(dims[a]) - 0.5f;
+    const float i0 = std::floor(u);
+    frac[a] = u - i0;
+    const int64_t n = static_cast<int64_t>(dims[a]);
+    const int64_t k = ((static_cast<int64_t>(i0) % n) + n) % n;
+    lo[a] = static_cast<uint32_t>(k);
+    hi[a] = static_cast<uint32_t>((k + 1) % n);
+  }
+  auto texel = [&](uint32_t x, uint32_t y, int ch) {
+    const size_t off = ((static_cast<size_t>(layer) * T.h + y) * T.w + x) * 4 + static_cast<size_t>(ch);
+    return static_cast<float>(T.rgba8[off]) / 255.0f;
+  };
+  auto lerp = [](float p, float q, float w) { return p + w * (q - p); };
+  for (int ch = 0; ch < 4; ++ch) {
+    const float top = lerp(texel(lo[0], lo[1], ch), texel(hi[0], lo[1], ch), frac[0]);
+    const float bottom = lerp(texel(lo[0], hi[1], ch), texel(hi[0], hi[1], ch), frac[0]);
+    out.c[ch] = lerp(top, bottom, frac[1]);
+  }
+  return out;
+}
+f2 texture_uv(const f2 uvs[2], uint32_t tex_coord) { return tex_coord == 1u ? uvs[1] : uvs[0]; }  // :173-178
+
+Mat decode_material(const cg_material& pm, float l, const TexArr& T, const f2 uvs[2]) {  // material.wesl:19-53
+  // baseColor (:180-191)
+  float color[4] = {pm.base_color_factor[0], pm.base_color_factor[1], pm.base_color_factor[2],
+                    pm.base_color_factor[3]};
+  const cg_tex_info& tb = pm.tex[CG_TEX_BASE_COLOR];
+  if (tb.index >= 0) {
+    Rgba t = sample_texture(T, tb.index, texture_uv(uvs, tb.tex_coord));
+    for (int k = 0; k < 4; ++k) color[k] *= t.c[k];
+  }
+  float refl = rgb_to_reflectance(mk(color[0], color[1], color[2]), l);
+  // roughness (:192-203) and metallic (:205-216): channels 1 and 2
+  const cg_tex_info& tmr = pm.tex[CG_TEX_METALLIC_ROUGHNESS];
+  float rough = pm.roughness_factor;
+  if (tmr.index >= 0) rough *= sample_texture(T, tmr.index, texture_uv(uvs, tmr.tex_coord)).c[1];
+  rough = clampf(rough, 0.0f, 1.0f);
+  float metal = pm.metallic_factor;
+  if (tmr.index >= 0) metal *= sample_texture(T, tmr.index, texture_uv(uvs, tmr.tex_coord)).c[2];
+  metal = clampf(metal, 0.0f, 1.0f);
+  // transmission (:218-229): channel 0
+  const cg_tex_info& tt = pm.tex[CG_TEX_TRANSMISSION];
+  float trans = pm.transmission_factor;
+  if (tt.index >= 0) trans *= sample_texture(T, tt.index, texture_uv(uvs, tt.tex_coord)).c[0];
+  trans = clampf(trans, 0.0f, 1.0f);
   float raw_ior = pm.ior;
   float real_ior = (metal == 0.0f && trans > 0.0f) ? dispersed_ior(raw_ior, pm.dispersion, l) : raw_ior;
   IOR ior{real_ior, metal * imag_ior(real_ior, refl)};
-  v3 em = mk(fmx(pm.emissive_factor[0] * pm.emissive_strength, 0.0f),
-             fmx(pm.emissive_factor[1] * pm.emissive_strength, 0.0f),
-             fmx(pm.emissive_factor[2] * pm.emissive_strength, 0.0f));
+  // emission (:250-262)
+  float strength[3] = {pm.emissive_factor[0], pm.emissive_factor[1], pm.emissive_factor[2]};
+  const cg_tex_info& te = pm.tex[CG_TEX_EMISSIVE];
+  if (te.index >= 0) {
+    Rgba t = sample_texture(T, te.index, texture_uv(uvs, te.tex_coord));
+    for (int k = 0; k < 3; ++k) strength[k] *= t.c[k];
+  }
+  v3 em = mk(fmx(strength[0] * pm.emissive_strength, 0.0f), fmx(strength[1] * pm.emissive_strength, 0.0f),
+             fmx(strength[2] * pm.emissive_strength, 0.0f));
   Mat m;
   m.diffuse = (1.0f - metal) * (1.0f - trans) * refl;
   m.alpha = rough * rough;
@@ -606,6 +681,9 @@ struct orc_scene {
   std::vector<cg_alias_entry> alias;
   std::vector<float> env;
   uint32_t env_w = 0, env_h = 0;
+  std::vector<uint8_t> tex;  // material texture array
+  uint32_t tex_w = 0, tex_h = 0, tex_layers = 0;
+  TexArr textures() const { return TexArr{tex.data(), tex_w, tex_h, tex_layers}; }
 };
 
 namespace {
@@ -935,7 +1013,8 @@ struct Ctx {  // shading.wesl:24-28 + sample.wesl:73-78
 Ctx forward_pass(const orc_scene& sc, v3 d, float l, const Surface& s, int depth, const Params& P,
                  uint32_t& rng) {  // shading.wesl:30-46
   const cg_material& pm = sc.mat[std::min<size_t>(s.material, sc.mat.size() - 1)];
-  Mat m = decode_material(pm, l);
+  const f2 uvs[2] = {s.uv0, s.uv1};
+  Mat m = decode_material(pm, l, sc.textures(), uvs);
   Counts6 c = sample_counts(sc, m, depth, P);
   Ctx x;
   auto add = [&](v3 dir, float coeff) {
@@ -1148,16 +1227,58 @@ void blas_build_tree(orc_scene& sc, uint32_t node_idx, uint32_t l, uint32_t r, s
   blas_build_tree(sc, left + 1, mid, r, ti, prim, verts, indices, tri_offset);
 }
 
+// Emissive buffer texture factor (emissive_buffer.h:135-212): the decoded
+// image (not the resized layer) at the triangle's centroid uv, nearest texel,
+// under the texture's own wrap modes.  Declared: a NaN coordinate reads as 0.
+float wrap_coord(float coord, int32_t wrap) {  // :191-204
+  float r;
+  if (wrap == CG_WRAP_CLAMP_TO_EDGE) {
+    r = coord;
+  } else if (wrap == CG_WRAP_MIRRORED_REPEAT) {
+    r = coord - std::floor(coord);
+    if (std::fmod(std::floor(coord), 2.0f) != 0.0f) r = 1.0f - r;  // odd tile
+  } else {
+    r = coord - std::floor(coord);
+  }
+  if (std::isnan(r)) r = 0.0f;
+  return std::min(std::max(r, 0.0f), 0.999999f);
+}
+void emissive_texel(const cg_texture& t, float u, float v, float* rgb) {  // SampleTexture, :170-189
+  uint32_t x = std::min(t.width - 1, static_cast<uint32_t>(wrap_coord(u, t.wrap_s) * static_cast<float>(t.width)));
+  uint32_t y = std::min(t.height - 1, static_cast<uint32_t>(wrap_coord(v, t.wrap_t) * static_cast<float>(t.height)));
+  const uint8_t* px = t.rgba8 + (static_cast<size_t>(y) * t.width + x) * 4;
+  for (int c = 0; c < 3; ++c) rgb[c] = static_cast<float>(px[c]) / 255.0f;
+}
+void emissive_texture_factor(const cg_material& m, const cg_vertex& v0, const cg_vertex& v1, const cg_vertex& v2,
+                             const cg_texture* textures, uint32_t texture_count, float* rgb) {  // :135-168
+  rgb[0] = rgb[1] = rgb[2] = 1.0f;
+  const cg_tex_info& ti = m.tex[CG_TEX_EMISSIVE];
+  if (ti.index < 0 || static_cast<uint32_t>(ti.index) >= texture_count) return;
+  const bool second = ti.tex_coord == 1u;
+  const float* a = second ? v0.uv1 : v0.uv0;
+  const float* b = second ? v1.uv1 : v1.uv0;
+  const float* c = second ? v2.uv1 : v2.uv0;
+  emissive_texel(textures[ti.index], (a[0] + b[0] + c[0]) / 3.0f, (a[1] + b[1] + c[1]) / 3.0f, rgb);
+}
+
 }  // namespace
 
 extern "C" {
 
 void orc_sincos(float x, float* s, float* c) { sincos_decl(x, *s, *c); }
 
+void orc_texture_sample(const uint8_t* rgba8, uint32_t w, uint32_t h, uint32_t layers, int32_t layer, float u,
+                        float v, float* out) {
+  Rgba r = sample_texture(TexArr{rgba8, w, h, layers}, layer, f2{u, v});
+  for (int k = 0; k < 4; ++k) out[k] = r.c[k];
+}
+
+void orc_emissive_texel(const cg_texture* t, float u, float v, float* rgb) { emissive_texel(*t, u, v, rgb); }
+
 orc_scene* orc_scene_build(const cg_vertex* vertices, uint32_t vertex_count, const uint32_t* indices,
                            uint32_t index_count, const cg_material* materials, uint32_t material_count,
-                           const cg_primitive* prims, uint32_t prim_count, const float* env_rgba32f,
-                           uint32_t env_w, uint32_t env_h) {
+                           const cg_primitive* prims, uint32_t prim_count, const cg_texture* textures,
+                           uint32_t texture_count, const float* env_rgba32f, uint32_t env_w, uint32_t env_h) {
   orc_scene* sc = new orc_scene();
   std::vector<cg_vertex> verts(vertices, vertices + vertex_count);
   std::vector<uint32_t> idx(indices, indices + index_count);
@@ -1296,7 +1417,9 @@ orc_scene* orc_scene_build(const cg_vertex* vertices, uint32_t vertex_count, con
       v3 p2 = glm_point(pr.world, mk(verts[idx[o + 2]].pos));
       float area = length(cross(p1 - p0, p2 - p0)) * 0.5f;
       if (area <= 0.0f) continue;
-      v3 em = base * mk(1.0f, 1.0f, 1.0f);
+      float f[3];
+      emissive_texture_factor(m, verts[idx[o]], verts[idx[o + 1]], verts[idx[o + 2]], textures, texture_count, f);
+      v3 em = base * mk(f[0], f[1], f[2]);
       float w = lum(em);
       if (w <= 0.0f) continue;
       cg_emissive_prim e{};
@@ -1318,6 +1441,32 @@ orc_scene* orc_scene_build(const cg_vertex* vertices, uint32_t vertex_count, con
   for (auto& e : sc->em) w.push_back(static_cast<double>(e.weight));
   sc->alias.resize(w.size());
   orc_alias_build(w.data(), static_cast<uint32_t>(w.size()), sc->alias.data());
+
+  // Material texture array (resource.cpp:471-546): every image resized
+  // nearest to the largest width and height (ResizeRgba8Nearest, :51-75).
+  if (texture_count > 0) {
+    uint32_t W = 1, H = 1;
+    for (uint32_t t = 0; t < texture_count; ++t) {
+      W = std::max(W, textures[t].width);
+      H = std::max(H, textures[t].height);
+    }
+    sc->tex_w = W;
+    sc->tex_h = H;
+    sc->tex_layers = texture_count;
+    sc->tex.resize(static_cast<size_t>(W) * H * 4 * texture_count);
+    for (uint32_t t = 0; t < texture_count; ++t) {
+      const cg_texture& img = textures[t];
+      uint8_t* dst = &sc->tex[static_cast<size_t>(t) * W * H * 4];
+      for (uint32_t y = 0; y < H; ++y) {
+        uint32_t sy = img.height == H ? y : std::min(img.height - 1, static_cast<uint32_t>(uint64_t{y} * img.height / H));
+        for (uint32_t x = 0; x < W; ++x) {
+          uint32_t sx = img.width == W ? x : std::min(img.width - 1, static_cast<uint32_t>(uint64_t{x} * img.width / W));
+          for (int c = 0; c < 4; ++c)
+            dst[(static_cast<size_t>(y) * W + x) * 4 + c] = img.rgba8[(static_cast<size_t>(sy) * img.width + sx) * 4 + c];
+        }
+      }
+    }
+  }
 
   // Environment (scene_data.h:57-98): given texture, else the fallback sky.
   if (env_rgba32f && env_w && env_h) {
@@ -1358,6 +1507,12 @@ orc_scene* orc_scene_wrap(const cg_scene_data* d) {
   sc->env.assign(d->env_rgba32f, d->env_rgba32f + static_cast<size_t>(d->env_width) * d->env_height * 4);
   sc->env_w = d->env_width;
   sc->env_h = d->env_height;
+  if (d->tex_layers && d->tex_rgba8) {
+    sc->tex.assign(d->tex_rgba8, d->tex_rgba8 + static_cast<size_t>(d->tex_width) * d->tex_height * 4 * d->tex_layers);
+    sc->tex_w = d->tex_width;
+    sc->tex_h = d->tex_height;
+    sc->tex_layers = d->tex_layers;
+  }
   return sc;
 }
 
@@ -1384,6 +1539,10 @@ void orc_scene_get(const orc_scene* s, cg_scene_data* o) {
   o->env_rgba32f = s->env.data();
   o->env_width = s->env_w;
   o->env_height = s->env_h;
+  o->tex_rgba8 = s->tex.empty() ? nullptr : s->tex.data();
+  o->tex_width = s->tex_w;
+  o->tex_height = s->tex_h;
+  o->tex_layers = s->tex_layers;
 }
 
 void orc_trace_closest(const orc_scene* s, const cg_ray* rays, uint32_t n, cg_hit* hits, float* uv,
@@ -1417,9 +1576,6 @@ void orc_trace_closest(const orc_scene* s, const cg_ray* rays, uint32_t n, cg_hi
 int orc_render(const orc_scene* s, const cg_conf* conf, const cg_camera* cam, uint32_t iter_count,
                cg_history_pixel* history, uint8_t* rgba8, uint32_t* path_hash, uint32_t* ray_count, uint32_t y0,
                uint32_t y1, int threads, int cull) {
-  for (const cg_material& m : s->mat)
-    for (int t : {CG_TEX_BASE_COLOR, CG_TEX_METALLIC_ROUGHNESS, CG_TEX_EMISSIVE, CG_TEX_TRANSMISSION})
-      if (m.tex[t].index >= 0) return -1;  // material textures: not restated
   const uint32_t W = conf->render_width, H = conf->render_height;
   Params P;
   P.D = static_cast<int>(std::min(std::max(conf->trace_depth, 1u), 64u));

@@ -18,7 +18,11 @@
  *   - tan(acos(c))^2 in the GGX masking terms = max(0, 1 - c^2) / c^2;
  *   - sin/cos of the sampling angles: Cody-Waite reduction and Taylor
  *     polynomials (orc_sincos; about 1 ulp, inside WGSL's 2^-11 bound);
- *   - exp/acos/atan2/pow(x, 1/2.4) are the C library's.
+ *   - exp/acos/atan2/pow(x, 1/2.4) are the C library's;
+ *   - material texture lookups: repeat-wrap in normalised coordinates, texel
+ *     centres at (i + 1/2) / size, texels k / 255, and the blend
+ *     lerp(lerp(T00, T10, a), lerp(T01, T11, a), b) with
+ *     lerp(p, q, w) = p + w * (q - p) (orc_texture_sample).
  */
 #ifndef CG_ORACLE_H_
 #define CG_ORACLE_H_
@@ -34,11 +38,13 @@ extern "C" {
 typedef struct orc_scene orc_scene;
 
 /* Scene preparation (src/pathtracing/scene_data.h:38-54): BLAS per primitive
- * object, TLAS, instances, emissive buffer, alias table, fallback sky. */
+ * object, TLAS, instances, emissive buffer (with its emissive-texture
+ * lookups), alias table, material texture array, fallback sky. */
 orc_scene* orc_scene_build(const cg_vertex* vertices, uint32_t vertex_count,
                            const uint32_t* indices, uint32_t index_count,
                            const cg_material* materials, uint32_t material_count,
                            const cg_primitive* prims, uint32_t prim_count,
+                           const cg_texture* textures, uint32_t texture_count,
                            const float* env_rgba32f, uint32_t env_w, uint32_t env_h);
 /* Take a SceneData built elsewhere (copied). */
 orc_scene* orc_scene_wrap(const cg_scene_data* sd);
@@ -67,6 +73,12 @@ int orc_render(const orc_scene* s, const cg_conf* conf, const cg_camera* cam,
                uint32_t y0, uint32_t y1, int threads, int cull);
 
 /* Unit-level pieces, for known-answer and closed-form checks. */
+/* Material texture lookup (bilinear, repeat, level 0) in an array of `layers`
+ * w x h RGBA8 images; out receives rgba. */
+void orc_texture_sample(const uint8_t* rgba8, uint32_t w, uint32_t h, uint32_t layers,
+                        int32_t layer, float u, float v, float* out);
+/* The emissive buffer's nearest lookup at (u, v) under t's wrap modes. */
+void orc_emissive_texel(const cg_texture* t, float u, float v, float* rgb);
 void orc_sincos(float x, float* s, float* c);
 uint32_t orc_rng_step(uint32_t state);
 float orc_rand_float(uint32_t* state);

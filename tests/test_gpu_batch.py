@@ -12,16 +12,18 @@ pytestmark = pytest.mark.gpu
 
 def _scenes(cg):
     out = {}
-    for name, params, seed in (("spheres", [8], 3), ("c1", [0.125], 2), ("c0", [], 1)):
+    for key, name, params, seed in (("spheres", "spheres", [8], 3), ("c1", "c1", [0.125], 2), ("c0", "c0", [], 1),
+                                    ("spheres_tex", "spheres", [8, 1, 64], 4)):
         s = cg.HostScene.generate(name, seed=seed, params=params)
-        out[name] = (s, s.scene_data())
+        out[key] = (s, s.scene_data())
     return out
 
 
 def _jobs():
     # (scene, width, height, spp, depth, camera perturbation)
     return [("spheres", 40, 24, 3, 4, 0.0), ("c1", 72, 40, 2, 8, 0.3), ("c0", 64, 36, 1, 2, 0.0),
-            ("spheres", 17, 9, 2, 3, 0.7), ("c1", 24, 24, 1, 5, -0.4), ("c0", 33, 20, 4, 3, 0.2)]
+            ("spheres", 17, 9, 2, 3, 0.7), ("c1", 24, 24, 1, 5, -0.4), ("c0", 33, 20, 4, 3, 0.2),
+            ("spheres_tex", 48, 32, 2, 4, 0.5)]
 
 
 def _camera(cg, desc, k):

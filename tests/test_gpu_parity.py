@@ -92,6 +92,8 @@ def _render_parity(cg, oracle, name, params, conf, frames=1, seed=13, accel=1):
     ("spheres", [8], 96, 64, 4, 4),
     ("c0", [], 320, 180, 1, 2),
     ("c1", [0.25], 128, 72, 2, 8),
+    ("spheres", [8, 1, 64], 96, 64, 4, 4),
+    ("c1", [0.25, 1, 256], 128, 72, 2, 8),
 ])
 def test_render_per_pixel(cg, oracle, name, params, w, h, spp, depth):
     conf = cg.Conf.make(w, h, spp, depth)

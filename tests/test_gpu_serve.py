@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("engine", [0, 1])
 def test_serve_results_equal_alone(cg, engine):
     scenes = [cg.HostScene.generate("spheres", seed=4, params=[6]),
-              cg.HostScene.generate("c1", seed=3, params=[0.125])]
+              cg.HostScene.generate("c1", seed=3, params=[0.125, 1, 256])]  # textured
     rng = np.random.default_rng(5)
     reqs = []
     for k in range(12):

@@ -93,7 +93,8 @@ def test_api_error_paths(cg):
     assert sc.describe()["prims"].shape[0] == 10
 
 
-@pytest.mark.parametrize("name,params", [("quad", []), ("spheres", [8]), ("c0", []), ("c1", [0.25])])
+@pytest.mark.parametrize("name,params", [("quad", []), ("spheres", [8]), ("c0", []), ("c1", [0.25]),
+                                         ("spheres", [8, 1, 64]), ("c1", [0.25, 1, 128])])
 def test_scene_prep_matches_oracle(cg, oracle, name, params):
     s = cg.HostScene.generate(name, seed=7, params=params)
     a = s.scene_data(share_blas=False, threads=8).arrays()
@@ -158,7 +159,7 @@ def test_ray_batch_halves(cg):
 @pytest.mark.parametrize("name,params,accel", [("c0", [], 1), ("c0", [], 0), ("c1", [0.25], 1), ("spheres", [8], 1),
                                                ("c2", [200000], 1)])
 def test_quantised_nodes_contain_exact_boxes(cg, name, params, accel):
-    """The device layout's 64-B quantised nodes decode (with the kernels' own
+    """The device layout's 128-B quantised 8-wide nodes decode (with the kernels' own
     float operations) to boxes that contain the exact child boxes, so the
     traversal visits a superset of the exact tree's nodes."""
     from crystalgraphics_amd import native
