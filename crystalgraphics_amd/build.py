@@ -44,9 +44,10 @@ def build_device() -> Path:
 def build_host() -> Path:
     LIB.mkdir(exist_ok=True)
     out = LIB / "libcrystal_graphics.so"
-    srcs = [str(PKG / "csrc/host" / f) for f in ("scene_prep.cpp", "scene_gen.cpp", "graphics.cpp", "serve.cpp")]
+    srcs = [str(PKG / "csrc/host" / f) for f in ("scene_prep.cpp", "scene_gen.cpp", "graphics.cpp", "serve.cpp",
+                                                              "gltf.cpp")]
     _run(["g++", *HOST_FLAGS, "-shared", "-o", str(out), *srcs,
-          f"-L{LIB}", "-lcg_device", "-Wl,-rpath,$ORIGIN", "-lpthread"])
+          f"-L{LIB}", "-lcg_device", "-Wl,-rpath,$ORIGIN", "-lpthread", "-lz"])
     return out
 
 

@@ -142,7 +142,14 @@ bool EnvStatus() { return static_cast<bool>(g_env); }
 expected<Scene, Error> LoadScene(std::filesystem::path file) {
   const std::string s = file.string();
   const std::string prefix = "procedural:";
-  if (s.rfind(prefix, 0) != 0) return unexpected<Error>(Error{"scene file not supported: " + s});
+  if (s.rfind(prefix, 0) != 0) {
+    const std::string ext = file.extension().string();
+    if (ext != ".gltf" && ext != ".glb") return unexpected<Error>(Error{"scene file not supported: " + s});
+    auto impl = std::make_unique<Scene::Impl>();
+    std::string err = LoadGltf(s, *impl);
+    if (!err.empty()) return unexpected<Error>(Error{"glTF " + s + ": " + err});
+    return MakeScene(file, std::move(impl));
+  }
   std::string name = s.substr(prefix.size());
   uint64_t seed = 1;
   auto q = name.find('?');

@@ -47,6 +47,8 @@ cg_material DefaultMaterial();
 // Returns an empty string on success, else the error.
 std::string GenerateScene(const std::string& name, std::uint64_t seed, const double* params, int nparams,
                           Scene::Impl& out);
+// glTF 2.0 scene file (.gltf or .glb); returns an empty string on success.
+std::string LoadGltf(const std::string& path, Scene::Impl& out);
 // Ray batch of the traversal microbenchmark over a generated "c2" scene:
 // half pinhole-camera rays on a grid, half surface-origin cosine rays.
 void GenerateRays(const Scene::Impl& scene, std::uint64_t seed, std::uint32_t n, cg_ray* out);

@@ -86,6 +86,8 @@ def test_api_error_paths(cg):
     with pytest.raises(cg.NativeError, match="already terminated"):
         cg.EnvTerm()
     with pytest.raises(cg.NativeError, match="not supported"):
+        cg.LoadScene("/nonexistent/scene.obj")
+    with pytest.raises(cg.NativeError, match="cannot read"):
         cg.LoadScene("/nonexistent/scene.gltf")
     with pytest.raises(cg.NativeError, match="unknown procedural scene"):
         cg.LoadScene("procedural:nope")
