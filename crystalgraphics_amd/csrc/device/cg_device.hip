@@ -2,6 +2,7 @@
 // (gfx950): scene upload into the device layout, the per-pixel compute pass
 // and the batch traversal entry points behind the C-ABI of cg_device.h.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -373,6 +374,38 @@ __global__ __launch_bounds__(kBlock, 4) void k_view_batch(const ViewArgs* __rest
   if (stats) flush_stats(stats, nrays, nshadow, st.nodes, st.tris, st.insts);
 }
 
+// Linear-BVH build of one BLAS (cg_lbvh.h): one thread per triangle, per
+// internal node, or per node of the current collapse level.
+__global__ __launch_bounds__(kBlock) void k_lbvh_morton(LbvhArrays A) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < A.n) lbvh_morton(A, i);
+}
+__global__ __launch_bounds__(kBlock) void k_lbvh_nodes(LbvhArrays A) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i + 1u < A.n) lbvh_node(A, i);
+}
+__global__ __launch_bounds__(kBlock) void k_lbvh_bounds(LbvhArrays A) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < A.n) lbvh_bounds(A, i);
+}
+__global__ __launch_bounds__(kBlock) void k_lbvh_collapse(LbvhArrays A, uint32_t count) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < count) lbvh_collapse(A, A.front_in[i]);
+}
+__global__ __launch_bounds__(kBlock) void k_lbvh_gather(const TriRec* __restrict__ tin, const uint4* __restrict__ vin,
+                                                        TriRec* __restrict__ tout, uint4* __restrict__ vout,
+                                                        const uint32_t* __restrict__ ids, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    tout[i] = tin[ids[i]];
+    vout[i] = vin[ids[i]];
+  }
+}
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // Plain streaming copy: 16 B per lane per iteration, grid-stride.
 __global__ __launch_bounds__(kBlock) void k_stream_copy(float4* __restrict__ dst, const float4* __restrict__ src,
                                                         size_t n) {
@@ -531,6 +564,116 @@ void release_scene(cg_scene* s) {
   dev_free(s->emit_order);
   dev_free(s->env);
   dev_free(s->tex);
+}
+
+// Builds one BLAS with the linear-BVH kernels: sorts its triangles, writes
+// their records into tris / vidx at the job's slots (from the source-order
+// copies), and returns its wide nodes (global refs from node_base) in `nodes`.
+int lbvh_build_device(cg_env* env, BlasJob& j, uint32_t node_base, const TriRec* tris_src, const uint4* vidx_src,
+                      TriRec* tris, uint4* vidx, DevBuf& nodes) {
+  const uint32_t n = static_cast<uint32_t>(j.boxes.size());
+  DevBuf prims, keys, keys2, ids, ids2, left, right, first, count, parent, box, flag, fa, fb, ctr, tmp;
+  auto release = [&] {
+    for (DevBuf* b : {&prims, &keys, &keys2, &ids, &ids2, &left, &right, &first, &count, &parent, &box, &flag, &fa,
+                      &fb, &ctr, &tmp})
+      dev_free(*b);
+  };
+  auto grid = [](uint32_t items) { return dim3((items + kBlock - 1u) / kBlock); };
+  int rc = CG_OK;
+  auto alloc = [&](DevBuf& b, size_t bytes) {
+    if (rc == CG_OK) rc = dev_alloc(env, b, bytes);
+  };
+  alloc(prims, n * sizeof(PrimBox));
+  for (DevBuf* b : {&keys, &keys2, &ids, &ids2}) alloc(*b, n * sizeof(uint32_t));
+  for (DevBuf* b : {&left, &right, &first, &count, &flag}) alloc(*b, (n - 1u) * sizeof(uint32_t));
+  alloc(parent, (2u * n - 1u) * sizeof(uint32_t));
+  alloc(box, (n - 1u) * sizeof(PrimBox));
+  alloc(fa, (n - 1u) * sizeof(uint2));
+  alloc(fb, (n - 1u) * sizeof(uint2));
+  alloc(ctr, 4 * sizeof(uint32_t));
+  alloc(nodes, (n - 1u) * sizeof(Node8Q));
+  size_t tmp_bytes = 0;
+  if (rc == CG_OK &&
+      hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, static_cast<uint32_t*>(keys.p), static_cast<uint32_t*>(keys2.p),
+                                         static_cast<uint32_t*>(ids.p), static_cast<uint32_t*>(ids2.p),
+                                         static_cast<int>(n), 0, 30, env->stream) != hipSuccess)
+    rc = fail(env, CG_ERR_HIP, "device BVH build: sort sizing failed");
+  alloc(tmp, tmp_bytes);
+  if (rc != CG_OK) {
+    release();
+    dev_free(nodes);
+    return rc;
+  }
+  LbvhArrays A{};
+  A.n = n;
+  A.prims = static_cast<const PrimBox*>(prims.p);
+  A.pad = j.pad;
+  lbvh_grid(j, A.cmin, A.cscale);
+  A.keys = static_cast<uint32_t*>(keys.p);
+  A.ids = static_cast<uint32_t*>(ids.p);
+  A.left = static_cast<uint32_t*>(left.p);
+  A.right = static_cast<uint32_t*>(right.p);
+  A.first = static_cast<uint32_t*>(first.p);
+  A.count = static_cast<uint32_t*>(count.p);
+  A.parent = static_cast<uint32_t*>(parent.p);
+  A.box = static_cast<PrimBox*>(box.p);
+  A.flag = static_cast<uint32_t*>(flag.p);
+  uint32_t* c = static_cast<uint32_t*>(ctr.p);  // [0] next frontier length, [1] wide nodes, [2] most children
+  A.front_out_n = c;
+  A.wide_n = c + 1;
+  A.max_children = c + 2;
+  A.nodes = static_cast<Node8Q*>(nodes.p);
+  A.exact = nullptr;
+  A.node_base = node_base;
+  A.slot_base = j.slot_base;
+  auto fin = [&](int code) {
+    release();
+    if (code != CG_OK) dev_free(nodes);
+    return code;
+  };
+  if (hipMemcpyAsync(prims.p, j.boxes.data(), n * sizeof(PrimBox), hipMemcpyHostToDevice, env->stream) != hipSuccess)
+    return fin(fail(env, CG_ERR_HIP, "device BVH build: box upload failed"));
+  hipLaunchKernelGGL(k_lbvh_morton, grid(n), dim3(kBlock), 0, env->stream, A);
+  if (hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, A.keys, static_cast<uint32_t*>(keys2.p), A.ids,
+                                         static_cast<uint32_t*>(ids2.p), static_cast<int>(n), 0, 30,
+                                         env->stream) != hipSuccess)
+    return fin(fail(env, CG_ERR_HIP, "device BVH build: sort failed"));
+  A.keys = static_cast<uint32_t*>(keys2.p);
+  A.ids = static_cast<uint32_t*>(ids2.p);
+  hipLaunchKernelGGL(k_fill_u32, grid(2u * n - 1u), dim3(kBlock), 0, env->stream, A.parent, kLbvhNone, 2u * n - 1u);
+  hipLaunchKernelGGL(k_fill_u32, grid(n - 1u), dim3(kBlock), 0, env->stream, A.flag, 0u, n - 1u);
+  hipLaunchKernelGGL(k_lbvh_nodes, grid(n - 1u), dim3(kBlock), 0, env->stream, A);
+  hipLaunchKernelGGL(k_lbvh_bounds, grid(n), dim3(kBlock), 0, env->stream, A);
+  const uint2 root = make_uint2(0u, 0u);
+  const uint32_t init[4] = {0u, 1u, 0u, 0u};
+  if (hipMemcpyAsync(fa.p, &root, sizeof root, hipMemcpyHostToDevice, env->stream) != hipSuccess ||
+      hipMemcpyAsync(c, init, sizeof init, hipMemcpyHostToDevice, env->stream) != hipSuccess)
+    return fin(fail(env, CG_ERR_HIP, "device BVH build: counter upload failed"));
+  uint2* cur = static_cast<uint2*>(fa.p);
+  uint2* nxt = static_cast<uint2*>(fb.p);
+  uint32_t width = 1, stack = 0, got[4] = {0u, 0u, 0u, 0u};
+  for (int level = 0; width > 0u; ++level) {
+    if (level >= 64) return fin(fail(env, CG_ERR_UNSUPPORTED, "device BVH build: tree deeper than 64 levels"));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, env->stream, c, 0u, 1u);      // next frontier length
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kBlock), 0, env->stream, c + 2, 0u, 1u);  // most children
+    A.front_in = cur;
+    A.front_out = nxt;
+    hipLaunchKernelGGL(k_lbvh_collapse, grid(width), dim3(kBlock), 0, env->stream, A, width);
+    if (hipMemcpyAsync(got, c, sizeof got, hipMemcpyDeviceToHost, env->stream) != hipSuccess ||
+        hipStreamSynchronize(env->stream) != hipSuccess)
+      return fin(fail(env, CG_ERR_HIP, "device BVH build: collapse level %d failed", level));
+    stack += got[2] > 0u ? got[2] - 1u : 0u;
+    width = got[0];
+    std::swap(cur, nxt);
+  }
+  hipLaunchKernelGGL(k_lbvh_gather, grid(n), dim3(kBlock), 0, env->stream, tris_src + j.slot_base,
+                     vidx_src + j.slot_base, tris + j.slot_base, vidx + j.slot_base, A.ids, n);
+  if (hipStreamSynchronize(env->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+    return fin(fail(env, CG_ERR_HIP, "device BVH build: kernels failed"));
+  j.root = node_base;
+  j.nodes = got[1];
+  j.stack = stack;
+  return fin(CG_OK);
 }
 
 // Resident grid of a kernel: blocks per CU at full occupancy x CUs, capped by
@@ -711,7 +854,8 @@ int cg_env_set_path_records(cg_env* env, int enable) {
 
 int cg_env_set_accel(cg_env* env, int mode) {
   if (!env) return fail(nullptr, CG_ERR_INVALID, "null env");
-  if (mode != CG_ACCEL_REFERENCE && mode != CG_ACCEL_REBUILT) return fail(env, CG_ERR_INVALID, "unknown accel mode %d", mode);
+  if (mode != CG_ACCEL_REFERENCE && mode != CG_ACCEL_REBUILT && mode != CG_ACCEL_DEVICE)
+    return fail(env, CG_ERR_INVALID, "unknown accel mode %d", mode);
   env->accel = mode;
   return CG_OK;
 }
@@ -750,20 +894,75 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
   *out = nullptr;
   Layout L;
   std::string err;
-  int rc = build_layout(sd, L, err, env->accel);
+  std::vector<BlasJob> jobs;
+  const bool on_device = env->accel == CG_ACCEL_DEVICE;
+  int rc;
+  if (on_device) {  // TLAS and small records on the host, every BLAS below on the device
+    rc = build_common(sd, L, err);
+    if (rc == CG_OK) rc = build_sah_accel(sd, L, err, &jobs);
+    if (rc == CG_OK) {
+      collapse_scene_bvh(L);
+      quantize_scene_bvh(L);
+      build_emitter_bvh(L);
+    }
+  } else {
+    rc = build_layout(sd, L, err, env->accel);
+  }
   if (rc != CG_OK) return fail(env, rc, "%s", err.c_str());
-  if (L.stack_depth > 4096u)
-    return fail(env, CG_ERR_UNSUPPORTED, "BVH needs a %u-entry traversal stack (max 4096)", L.stack_depth);
   cg_scene* s = new cg_scene();
   uint64_t total = 0;
   auto up = [&](int r) {
     if (r != CG_OK && rc == CG_OK) rc = r;
   };
-  up(upload_vec(env, s->bvh, L.bvhq, total));
+  if (on_device) {
+    // source-order triangle records, then one build per BLAS into node
+    // ranges after the host-built TLAS nodes
+    DevBuf tsrc, vsrc;
+    up(dev_upload(env, tsrc, L.tris.data(), L.tris.size() * sizeof(TriRec)));
+    up(dev_upload(env, vsrc, L.vidx.data(), L.vidx.size() * sizeof(uint4)));
+    up(dev_alloc(env, s->tris, L.tris.size() * sizeof(TriRec)));
+    up(dev_alloc(env, s->vidx, L.vidx.size() * sizeof(uint4)));
+    if (rc == CG_OK && L.tris.size() &&
+        (hipMemcpyAsync(s->tris.p, tsrc.p, tsrc.bytes, hipMemcpyDeviceToDevice, env->stream) != hipSuccess ||
+         hipMemcpyAsync(s->vidx.p, vsrc.p, vsrc.bytes, hipMemcpyDeviceToDevice, env->stream) != hipSuccess))
+      rc = fail(env, CG_ERR_HIP, "scene upload failed");
+    std::vector<DevBuf> built(jobs.size());
+    uint64_t node_count = L.bvhq.size();
+    for (size_t k = 0; k < jobs.size() && rc == CG_OK; ++k) {
+      if (node_count >= CG_REF_NONE) {
+        rc = fail(env, CG_ERR_UNSUPPORTED, "too many BVH nodes");
+        break;
+      }
+      rc = lbvh_build_device(env, jobs[k], static_cast<uint32_t>(node_count), static_cast<const TriRec*>(tsrc.p),
+                             static_cast<const uint4*>(vsrc.p), static_cast<TriRec*>(s->tris.p),
+                             static_cast<uint4*>(s->vidx.p), built[k]);
+      node_count += jobs[k].nodes;
+    }
+    if (rc == CG_OK && node_count >= CG_REF_NONE) rc = fail(env, CG_ERR_UNSUPPORTED, "too many BVH nodes");
+    up(dev_alloc(env, s->bvh, node_count * sizeof(Node8Q)));
+    if (rc == CG_OK && !L.bvhq.empty() &&
+        hipMemcpyAsync(s->bvh.p, L.bvhq.data(), L.bvhq.size() * sizeof(Node8Q), hipMemcpyHostToDevice,
+                       env->stream) != hipSuccess)
+      rc = fail(env, CG_ERR_HIP, "scene upload failed");
+    for (size_t k = 0; k < jobs.size() && rc == CG_OK; ++k)
+      if (hipMemcpyAsync(static_cast<Node8Q*>(s->bvh.p) + jobs[k].root, built[k].p, jobs[k].nodes * sizeof(Node8Q),
+                         hipMemcpyDeviceToDevice, env->stream) != hipSuccess)
+        rc = fail(env, CG_ERR_HIP, "scene upload failed");
+    if (hipStreamSynchronize(env->stream) != hipSuccess && rc == CG_OK) rc = fail(env, CG_ERR_HIP, "scene upload failed");
+    for (DevBuf& b : built) dev_free(b);
+    dev_free(tsrc);
+    dev_free(vsrc);
+    total += node_count * sizeof(Node8Q) + L.tris.size() * sizeof(TriRec) + L.vidx.size() * sizeof(uint4);
+    if (rc == CG_OK) lbvh_finish(L, jobs);
+  } else {
+    up(upload_vec(env, s->bvh, L.bvhq, total));
+    up(upload_vec(env, s->tris, L.tris, total));
+    up(upload_vec(env, s->vidx, L.vidx, total));
+  }
+  if (rc == CG_OK && L.stack_depth > 4096u)
+    rc = fail(env, CG_ERR_UNSUPPORTED, "BVH needs a %u-entry traversal stack (max 4096)", L.stack_depth);
   up(upload_vec(env, s->nodes, L.nodes, total));
-  up(upload_vec(env, s->tris, L.tris, total));
   up(upload_vec(env, s->insts, L.insts, total));
-  up(upload_vec(env, s->vidx, L.vidx, total));
   up(upload_vec(env, s->vattr, L.vattr, total));
   up(upload_vec(env, s->mats, L.mats, total));
   up(upload_vec(env, s->emit, L.emit, total));
@@ -819,9 +1018,26 @@ int cg_layout_check(const cg_scene_data* sd, int accel, uint64_t* nodes, uint32_
   std::string err;
   int rc = build_layout(sd, L, err, accel);
   if (rc != CG_OK) return fail(nullptr, rc, "%s", err.c_str());
+  if (L.bvh.size() != L.bvhq.size()) return fail(nullptr, CG_ERR_INVALID, "node arrays differ in length");
   for (size_t i = 0; i < L.bvh.size(); ++i)
     if (!quantized_contains(L.bvh[i], L.bvhq[i]))
       return fail(nullptr, CG_ERR_INVALID, "quantised node %zu does not contain its children's boxes", i);
+  // every triangle slot in exactly one leaf (of a node, or an instance's root)
+  std::vector<uint8_t> seen(L.tris.size(), 0);
+  auto leaf = [&](uint32_t r) {
+    if (r == CG_REF_NONE || (r & CG_REF_LEAF) == 0u) return true;
+    const uint32_t slot = r & CG_REF_SLOT_MASK, cnt = ((r >> CG_REF_COUNT_SHIFT) & 3u) + 1u;
+    for (uint32_t k = 0; k < cnt; ++k)
+      if (slot + k >= seen.size() || seen[slot + k]++) return false;
+    return true;
+  };
+  for (const Node8& n : L.bvh)
+    for (int i = 0; i < CG_WIDE; ++i)
+      if (!leaf(n.ref[i])) return fail(nullptr, CG_ERR_INVALID, "a triangle slot is in two leaves or out of range");
+  for (const InstRec& in : L.insts)
+    if (!leaf(in.root)) return fail(nullptr, CG_ERR_INVALID, "a triangle slot is in two leaves or out of range");
+  for (size_t k = 0; k < seen.size(); ++k)
+    if (!seen[k]) return fail(nullptr, CG_ERR_INVALID, "triangle slot %zu is in no leaf", k);
   if (nodes) *nodes = L.bvh.size();
   if (stack_depth) *stack_depth = L.stack_depth;
   return CG_OK;

@@ -1,0 +1,238 @@
+// cg_lbvh.h — building a BLAS on the device: a linear BVH (Karras, "Maximizing
+// Parallelism in the Construction of BVHs, Octrees, and k-d Trees", HPG 2012)
+// over 30-bit Morton codes of the triangle box centres, bottom-up boxes,
+// then a top-down collapse into 8-wide nodes (a node opens its child of
+// largest surface area until it has eight; subtrees of at most CG_LEAF_MAX
+// triangles become leaves, which are contiguous ranges of the sorted
+// triangles) quantised as the host build's nodes are.
+//
+// Each step is a per-thread function over plain arrays: the kernels in
+// cg_device.hip run them one thread per item, and lbvh_build_host runs the
+// same functions in loops, so the layout check on the host covers the exact
+// code the device runs.  The device sorts with a stable radix sort and the
+// host with a stable sort on the same keys: the same order.
+#ifndef CG_LBVH_H_
+#define CG_LBVH_H_
+
+#include "cg_layout.h"
+#include "cg_quant.h"
+
+namespace cg {
+
+struct PrimBox {
+  float lo[3], hi[3];
+};
+
+// Child encoding in the binary tree: internal node index, or leaf (sorted
+// triangle position) with the top bit set.
+constexpr uint32_t kLbvhLeaf = 0x80000000u;
+constexpr uint32_t kLbvhNone = 0xFFFFFFFFu;
+
+struct LbvhArrays {
+  uint32_t n;                 // triangles, >= 2
+  const PrimBox* prims;       // unsorted triangle boxes (unpadded)
+  float pad;                  // added to every triangle box
+  float cmin[3], cscale[3];   // centre -> [0, 1024) per axis
+  uint32_t* keys;             // n Morton codes (sorted in place by the driver)
+  uint32_t* ids;              // n triangle ids (sorted with the keys)
+  uint32_t* left;             // n - 1 internal nodes: children
+  uint32_t* right;
+  uint32_t* first;            // first sorted triangle of the node's range
+  uint32_t* count;            // triangles in the node's range
+  uint32_t* parent;           // n - 1 internal + n leaves (leaf i at n - 1 + i)
+  PrimBox* box;               // n - 1 internal node boxes (padded)
+  uint32_t* flag;             // n - 1 arrival counters, zero before the bounds pass
+  // collapse
+  uint2* front_in;            // (binary internal node, wide node) pairs
+  uint2* front_out;
+  uint32_t* front_out_n;
+  uint32_t* wide_n;           // wide nodes allocated so far
+  uint32_t* max_children;     // most children of a node built in this pass
+  Node8Q* nodes;              // wide nodes, local indices
+  Node8* exact;               // nullable: the unquantised nodes too (host checks)
+  uint32_t node_base;         // global index of local wide node 0
+  uint32_t slot_base;         // global triangle slot of sorted triangle 0
+};
+
+CG_QHD uint32_t lbvh_clz(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return x == 0u ? 32u : static_cast<uint32_t>(__clz(static_cast<int>(x)));
+#else
+  return x == 0u ? 32u : static_cast<uint32_t>(__builtin_clz(x));
+#endif
+}
+CG_QHD uint32_t spread3(uint32_t v) {  // 10 bits -> every third bit of 30
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+// Step 1 (per triangle i): Morton code of the box centre.
+CG_QHD void lbvh_morton(const LbvhArrays& A, uint32_t i) {
+  const PrimBox& p = A.prims[i];
+  uint32_t q[3];
+  for (int a = 0; a < 3; ++a) {
+    const float c = 0.5f * (p.lo[a] + p.hi[a]);
+    const float x = fminf(fmaxf((c - A.cmin[a]) * A.cscale[a], 0.0f), 1023.0f);
+    q[a] = static_cast<uint32_t>(x);
+  }
+  A.keys[i] = (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
+  A.ids[i] = i;
+}
+
+// Common-prefix length of sorted keys i and j (-1 outside the array; equal
+// keys are told apart by their positions).
+CG_QHD int lbvh_delta(const LbvhArrays& A, int i, int j) {
+  if (j < 0 || j >= static_cast<int>(A.n)) return -1;
+  const uint32_t ki = A.keys[i], kj = A.keys[j];
+  if (ki == kj) return 32 + static_cast<int>(lbvh_clz(static_cast<uint32_t>(i) ^ static_cast<uint32_t>(j)));
+  return static_cast<int>(lbvh_clz(ki ^ kj));
+}
+
+// Step 2 (per internal node i < n - 1): its range, split and children.
+CG_QHD void lbvh_node(const LbvhArrays& A, uint32_t iu) {
+  const int i = static_cast<int>(iu);
+  const int d = (lbvh_delta(A, i, i + 1) - lbvh_delta(A, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = lbvh_delta(A, i, i - d);
+  int lmax = 2;
+  while (lbvh_delta(A, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (lbvh_delta(A, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = lbvh_delta(A, i, j);
+  int s = 0;
+  for (int div = 2;; div *= 2) {
+    const int t = (l + div - 1) / div;
+    if (lbvh_delta(A, i, i + (s + t) * d) > dnode) s += t;
+    if (t <= 1) break;
+  }
+  const int gamma = i + s * d + (d < 0 ? -1 : 0);
+  const int lo = i < j ? i : j, hi = i < j ? j : i;
+  const uint32_t lc = lo == gamma ? (kLbvhLeaf | static_cast<uint32_t>(gamma)) : static_cast<uint32_t>(gamma);
+  const uint32_t rc = hi == gamma + 1 ? (kLbvhLeaf | static_cast<uint32_t>(gamma + 1)) : static_cast<uint32_t>(gamma + 1);
+  A.left[i] = lc;
+  A.right[i] = rc;
+  A.first[i] = static_cast<uint32_t>(lo);
+  A.count[i] = static_cast<uint32_t>(hi - lo + 1);
+  const uint32_t leaf0 = A.n - 1u;
+  A.parent[(lc & kLbvhLeaf) ? leaf0 + (lc & ~kLbvhLeaf) : lc] = iu;
+  A.parent[(rc & kLbvhLeaf) ? leaf0 + (rc & ~kLbvhLeaf) : rc] = iu;
+}
+
+CG_QHD PrimBox lbvh_leaf_box(const LbvhArrays& A, uint32_t k) {
+  const PrimBox& p = A.prims[A.ids[k]];
+  PrimBox b;
+  for (int a = 0; a < 3; ++a) {
+    b.lo[a] = p.lo[a] - A.pad;
+    b.hi[a] = p.hi[a] + A.pad;
+  }
+  return b;
+}
+CG_QHD PrimBox lbvh_child_box(const LbvhArrays& A, uint32_t c) {
+  return (c & kLbvhLeaf) ? lbvh_leaf_box(A, c & ~kLbvhLeaf) : A.box[c];
+}
+CG_QHD PrimBox lbvh_union(const PrimBox& x, const PrimBox& y) {
+  PrimBox b;
+  for (int a = 0; a < 3; ++a) {
+    b.lo[a] = fminf(x.lo[a], y.lo[a]);
+    b.hi[a] = fmaxf(x.hi[a], y.hi[a]);
+  }
+  return b;
+}
+
+// Step 3: bounds, from every leaf towards the root; the second arrival at a
+// node computes its box.  The atomic increment orders the first arrival's
+// box store before the second's loads (acquire-release, device scope).
+CG_QHD uint32_t lbvh_arrive(uint32_t* flag) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __hip_atomic_fetch_add(flag, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+#else
+  return (*flag)++;
+#endif
+}
+CG_QHD void lbvh_bounds(const LbvhArrays& A, uint32_t leaf) {
+  uint32_t p = A.parent[A.n - 1u + leaf];
+  // a Karras tree is at most 62 levels deep (30 key bits, 32 position bits)
+  for (int level = 0; p != kLbvhNone && level < 64; ++level) {
+    if (lbvh_arrive(&A.flag[p]) == 0u) return;
+    A.box[p] = lbvh_union(lbvh_child_box(A, A.left[p]), lbvh_child_box(A, A.right[p]));
+#if defined(__HIP_DEVICE_COMPILE__)
+    __threadfence();
+#endif
+    p = A.parent[p];
+  }
+}
+
+// Step 4 (per frontier entry): one 8-wide node.  The driver sizes the node
+// array for n - 1 nodes: every wide node comes from a distinct internal node.
+CG_QHD uint32_t lbvh_alloc(uint32_t* counter) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return atomicAdd(counter, 1u);
+#else
+  return (*counter)++;
+#endif
+}
+CG_QHD void lbvh_collapse(const LbvhArrays& A, uint2 entry) {
+  uint32_t ch[CG_WIDE];
+  PrimBox bx[CG_WIDE];
+  int n = 2;
+  ch[0] = A.left[entry.x];
+  ch[1] = A.right[entry.x];
+  bx[0] = lbvh_child_box(A, ch[0]);
+  bx[1] = lbvh_child_box(A, ch[1]);
+  auto openable = [&](uint32_t c) { return (c & kLbvhLeaf) == 0u && A.count[c] > CG_LEAF_MAX; };
+  while (n < CG_WIDE) {
+    int best = -1;
+    float ba = -1.0f;
+    for (int k = 0; k < n; ++k)
+      if (openable(ch[k])) {
+        const float x = bx[k].hi[0] - bx[k].lo[0], y = bx[k].hi[1] - bx[k].lo[1], z = bx[k].hi[2] - bx[k].lo[2];
+        const float a = x * y + y * z + z * x;
+        if (a > ba) {
+          ba = a;
+          best = k;
+        }
+      }
+    if (best < 0) break;
+    const uint32_t c = ch[best];
+    ch[best] = A.left[c];
+    bx[best] = lbvh_child_box(A, ch[best]);
+    ch[n] = A.right[c];
+    bx[n] = lbvh_child_box(A, ch[n]);
+    ++n;
+  }
+  Node8 nd;
+  for (int k = 0; k < CG_WIDE; ++k) {
+    for (int a = 0; a < 3; ++a) {
+      nd.lo[a][k] = k < n ? bx[k].lo[a] : __builtin_inff();
+      nd.hi[a][k] = k < n ? bx[k].hi[a] : -__builtin_inff();
+    }
+    nd.ref[k] = CG_REF_NONE;
+  }
+  for (int k = 0; k < n; ++k) {
+    const uint32_t c = ch[k];
+    if (c & kLbvhLeaf) {
+      nd.ref[k] = CG_MAKE_LEAF(A.slot_base + (c & ~kLbvhLeaf), 1u);
+    } else if (A.count[c] <= CG_LEAF_MAX) {
+      nd.ref[k] = CG_MAKE_LEAF(A.slot_base + A.first[c], A.count[c]);
+    } else {
+      const uint32_t w = lbvh_alloc(A.wide_n);
+      nd.ref[k] = A.node_base + w;
+      A.front_out[lbvh_alloc(A.front_out_n)] = make_uint2(c, w);
+    }
+  }
+  A.nodes[entry.y] = quantize_node(nd);
+  if (A.exact) A.exact[entry.y] = nd;
+#if defined(__HIP_DEVICE_COMPILE__)
+  atomicMax(A.max_children, static_cast<uint32_t>(n));
+#else
+  if (static_cast<uint32_t>(n) > *A.max_children) *A.max_children = static_cast<uint32_t>(n);
+#endif
+}
+
+}  // namespace cg
+
+#endif  // CG_LBVH_H_

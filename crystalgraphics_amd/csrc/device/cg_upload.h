@@ -17,6 +17,8 @@
 #include "../../../include/cg_device.h"
 #include "cg_core.h"
 #include "cg_layout.h"
+#include "cg_lbvh.h"
+#include "cg_quant.h"
 
 namespace cg {
 
@@ -637,7 +639,22 @@ inline uint32_t emit_pairs(const SahTree& t, std::vector<PairNode>& out, uint32_
 //    BLAS whose triangles carry their instance index -- their local rays equal
 //    the world rays, so the same triangle tests run on the same rays;
 //  * a SAH TLAS over the remaining instances and the merged BLAS.
-inline int build_sah_accel(const cg_scene_data* sd, Layout& L, std::string& err) {
+// A BLAS left to the linear-BVH build (CG_ACCEL_DEVICE): its triangles sit in
+// L.tris / L.vidx at [slot_base, slot_base + boxes.size()) in source order
+// until the build sorts them; instances refer to it through the placeholder
+// root CG_REF_INST | job index (an instance ref is never a BLAS root) until
+// its nodes are placed.
+struct BlasJob {
+  uint32_t slot_base = 0;
+  std::vector<PrimBox> boxes;
+  float pad = 0.0f;
+  uint32_t root = CG_REF_NONE;   // global root ref once built
+  uint32_t nodes = 0;            // wide nodes
+  uint32_t stack = 0;            // deepest stack the BLAS needs
+};
+
+inline int build_sah_accel(const cg_scene_data* sd, Layout& L, std::string& err,
+                           std::vector<BlasJob>* jobs = nullptr) {
   const uint32_t nb = sd->blas_node_count, ni = sd->instance_count, nt = sd->tlas_node_count;
   // World boxes of the instances: the reference TLAS leaves.
   std::vector<int> have_box(ni, 0);
@@ -682,6 +699,23 @@ inline int build_sah_accel(const cg_scene_data* sd, Layout& L, std::string& err)
       for (int r = 0; r < 4; ++r)
         if (m[4 * c + r] != (c == r ? 1.0f : 0.0f)) return false;
     return true;
+  };
+  // A BLAS for the device build: source-order triangles, a placeholder root;
+  // four triangles or fewer make one leaf.
+  auto add_job = [&](const std::vector<SahTree::Prim>& prims, float pad) -> uint32_t {
+    const uint32_t base = static_cast<uint32_t>(L.tris.size()) - static_cast<uint32_t>(prims.size());
+    if (prims.size() <= CG_LEAF_MAX) return CG_MAKE_LEAF(base, static_cast<uint32_t>(prims.size()));
+    BlasJob j;
+    j.slot_base = base;
+    j.pad = pad;
+    j.boxes.resize(prims.size());
+    for (size_t k = 0; k < prims.size(); ++k)
+      for (int a = 0; a < 3; ++a) {
+        j.boxes[k].lo[a] = prims[k].lo[a];
+        j.boxes[k].hi[a] = prims[k].hi[a];
+      }
+    jobs->push_back(std::move(j));
+    return CG_REF_INST | static_cast<uint32_t>(jobs->size() - 1);
   };
   auto tri_prim = [&](uint32_t s) {
     const cg_index& ix = sd->indices[s];
@@ -737,6 +771,16 @@ inline int build_sah_accel(const cg_scene_data* sd, Layout& L, std::string& err)
       const auto& sl = root_slots[ri];
       std::vector<SahTree::Prim> prims(sl.size());
       for (size_t k = 0; k < sl.size(); ++k) prims[k] = tri_prim(sl[k]);
+      if (jobs) {
+        for (uint32_t sidx : sl) {
+          L.tris.push_back(make_tri(sd, sidx, CG_MISS_ID));
+          const cg_index& ix = sd->indices[sidx];
+          L.vidx.push_back(make_uint4(ix.i0, ix.i1, ix.i2, 0u));
+        }
+        built_root[ri] = add_job(prims, pad_of(prims));
+        L.insts[i].root = built_root[ri];
+        continue;
+      }
       SahTree t;
       t.build(prims, CG_LEAF_MAX);
       const uint32_t slot_base = static_cast<uint32_t>(L.tris.size());
@@ -768,30 +812,51 @@ inline int build_sah_accel(const cg_scene_data* sd, Layout& L, std::string& err)
         prims.push_back(tri_prim(s));
         tri_inst.emplace_back(s, i);
       }
-    SahTree t;
-    t.build(prims, CG_LEAF_MAX);
-    const uint32_t slot_base = static_cast<uint32_t>(L.tris.size());
-    for (uint32_t id : t.order) {
-      L.tris.push_back(make_tri(sd, tri_inst[id].first, tri_inst[id].second));
-      const cg_index& ix = sd->indices[tri_inst[id].first];
-      L.vidx.push_back(make_uint4(ix.i0, ix.i1, ix.i2, 0u));
-    }
     const float pad = pad_of(prims);
     InstRec w{};
     for (int k = 0; k < 16; ++k) w.inv[k] = (k % 5 == 0) ? 1.0f : 0.0f;
-    w.root = emit_pairs(t, L.nodes, slot_base, nullptr, pad);
     w.material = 0u;
-    const uint32_t wi = static_cast<uint32_t>(L.insts.size());
-    L.insts.push_back(w);
     SahTree::Prim box;
+    if (jobs) {
+      for (const auto& ti : tri_inst) {
+        L.tris.push_back(make_tri(sd, ti.first, ti.second));
+        const cg_index& ix = sd->indices[ti.first];
+        L.vidx.push_back(make_uint4(ix.i0, ix.i1, ix.i2, 0u));
+      }
+      w.root = add_job(prims, pad);
+      for (int k = 0; k < 3; ++k) {
+        box.lo[k] = INFINITY;
+        box.hi[k] = -INFINITY;
+        for (const auto& p : prims) {
+          box.lo[k] = std::min(box.lo[k], p.lo[k]);
+          box.hi[k] = std::max(box.hi[k], p.hi[k]);
+        }
+      }
+    } else {
+      SahTree t;
+      t.build(prims, CG_LEAF_MAX);
+      const uint32_t slot_base = static_cast<uint32_t>(L.tris.size());
+      for (uint32_t id : t.order) {
+        L.tris.push_back(make_tri(sd, tri_inst[id].first, tri_inst[id].second));
+        const cg_index& ix = sd->indices[tri_inst[id].first];
+        L.vidx.push_back(make_uint4(ix.i0, ix.i1, ix.i2, 0u));
+      }
+      w.root = emit_pairs(t, L.nodes, slot_base, nullptr, pad);
+      for (int k = 0; k < 3; ++k) {
+        box.lo[k] = t.nodes[0].lo[k];
+        box.hi[k] = t.nodes[0].hi[k];
+      }
+      max_blas_depth = std::max(max_blas_depth, t.depth);
+    }
     for (int k = 0; k < 3; ++k) {
-      box.lo[k] = t.nodes[0].lo[k] - pad;
-      box.hi[k] = t.nodes[0].hi[k] + pad;
+      box.lo[k] -= pad;
+      box.hi[k] += pad;
       box.c[k] = 0.5f * (box.lo[k] + box.hi[k]);
     }
+    const uint32_t wi = static_cast<uint32_t>(L.insts.size());
+    L.insts.push_back(w);
     tlas_prims.push_back(box);
     tlas_ids.push_back(wi);
-    max_blas_depth = std::max(max_blas_depth, t.depth);
   }
   if (L.tris.size() >= CG_REF_SLOT_MASK) return set_error(err, CG_ERR_INVALID, "too many triangles");
   // TLAS over instance boxes (already padded), one instance per leaf.
@@ -901,7 +966,9 @@ inline uint32_t stack_bound_wide(const std::vector<Node8>& bvh, const std::vecto
   if (r == CG_REF_NONE || (r & CG_REF_LEAF) != 0u) return 0u;
   if ((r & CG_REF_INST) != 0u) {
     const InstRec& in = insts[r & CG_REF_INDEX_MASK];
-    return in.root == CG_REF_NONE ? 0u : 1u + stack_bound_wide(bvh, insts, in.root, memo);
+    if (in.root == CG_REF_NONE) return 0u;
+    if ((in.root & (CG_REF_LEAF | CG_REF_INST)) == CG_REF_INST) return 1u;  // a BLAS still to be built: bound added later
+    return 1u + stack_bound_wide(bvh, insts, in.root, memo);
   }
   if (memo[r] != ~0u) return memo[r];
   const Node8& n = bvh[r];
@@ -926,75 +993,6 @@ inline void collapse_scene_bvh(Layout& L) {
   L.nodes.shrink_to_fit();
 }
 
-// ---------------------------------------------------------------------------
-// Quantised nodes.  The decoder (trav_step) computes a bound as
-// origin + float(q) * step with step = 2^(e - 127): one exact product and one
-// rounded sum.  The encoder picks per axis the smallest step whose 255th grid
-// line reaches the node's upper bound, then per child the largest q whose
-// decoded value is <= the child's lower bound and the smallest q whose decoded
-// value is >= its upper bound, evaluating the decoder's own expression.
-inline float q_step(uint32_t biased) {
-  const uint32_t bits = biased << 23;
-  float f;
-  std::memcpy(&f, &bits, sizeof f);
-  return f;
-}
-inline float q_decode(float origin, uint32_t q, float step) { return origin + static_cast<float>(q) * step; }
-
-inline Node8Q quantize_node(const Node8& n) {
-  Node8Q out;
-  std::memset(&out, 0, sizeof out);
-  int used[CG_WIDE];
-  int nu = 0;
-  for (int i = 0; i < CG_WIDE; ++i) {
-    out.ref[i] = n.ref[i];
-    if (n.ref[i] != CG_REF_NONE) used[nu++] = i;
-  }
-  uint32_t exps = 0;
-  for (int a = 0; a < 3; ++a) {
-    float blo = __builtin_inff(), bhi = -__builtin_inff();
-    for (int k = 0; k < nu; ++k) {
-      blo = std::min(blo, n.lo[a][used[k]]);
-      bhi = std::max(bhi, n.hi[a][used[k]]);
-    }
-    if (nu == 0) blo = bhi = 0.0f;
-    out.origin[a] = blo;
-    const double extent = static_cast<double>(bhi) - static_cast<double>(blo);
-    int be = 1;
-    if (extent > 0.0) be = std::max(1, static_cast<int>(std::floor(std::log2(extent / 255.0))) + 127 - 1);
-    while (be < 254 && q_decode(blo, 255u, q_step(static_cast<uint32_t>(be))) < bhi) ++be;
-    const float step = q_step(static_cast<uint32_t>(be));
-    exps |= static_cast<uint32_t>(be) << (8 * a);
-    for (int k = 0; k < nu; ++k) {
-      const int i = used[k];
-      const double fl = std::floor((static_cast<double>(n.lo[a][i]) - blo) / step);
-      uint32_t ql = static_cast<uint32_t>(std::min(255.0, std::max(0.0, fl)));
-      while (ql > 0u && q_decode(blo, ql, step) > n.lo[a][i]) --ql;
-      const double fh = std::ceil((static_cast<double>(n.hi[a][i]) - blo) / step);
-      uint32_t qh = static_cast<uint32_t>(std::min(255.0, std::max(0.0, fh)));
-      while (qh < 255u && q_decode(blo, qh, step) < n.hi[a][i]) ++qh;
-      out.qlo[a][i] = static_cast<uint8_t>(ql);
-      out.qhi[a][i] = static_cast<uint8_t>(qh);
-    }
-  }
-  out.exps = exps;
-  return out;
-}
-
-// Every decoded child box contains the exact one.
-inline bool quantized_contains(const Node8& n, const Node8Q& q) {
-  for (int i = 0; i < CG_WIDE; ++i) {
-    if (n.ref[i] != q.ref[i]) return false;
-    if (n.ref[i] == CG_REF_NONE) continue;
-    for (int a = 0; a < 3; ++a) {
-      const float step = q_step((q.exps >> (8 * a)) & 0xFFu);
-      if (!(q_decode(q.origin[a], q.qlo[a][i], step) <= n.lo[a][i])) return false;
-      if (!(q_decode(q.origin[a], q.qhi[a][i], step) >= n.hi[a][i])) return false;
-    }
-  }
-  return true;
-}
-
 inline void quantize_scene_bvh(Layout& L) {
   L.bvhq.resize(L.bvh.size());
   const size_t n = L.bvh.size();
@@ -1011,9 +1009,126 @@ inline void quantize_scene_bvh(Layout& L) {
   for (auto& th : pool) th.join();
 }
 
+// The Morton grid of a BLAS: box centres -> [0, 1024) per axis.
+inline void lbvh_grid(const BlasJob& j, float* cmin, float* cscale) {
+  for (int a = 0; a < 3; ++a) {
+    float lo = INFINITY, hi = -INFINITY;
+    for (const PrimBox& b : j.boxes) {
+      const float c = 0.5f * (b.lo[a] + b.hi[a]);
+      lo = std::min(lo, c);
+      hi = std::max(hi, c);
+    }
+    cmin[a] = lo;
+    cscale[a] = hi > lo ? 1024.0f / (hi - lo) : 0.0f;
+  }
+}
+
+// The linear-BVH build of one BLAS on the host, running the device build's
+// per-thread steps in loops.  Appends its wide nodes (global indices from
+// node_base) to `nodes` (and the exact ones to `exact`), and returns the
+// sorted order of its triangles.
+inline std::vector<uint32_t> lbvh_build_host(BlasJob& j, uint32_t node_base, std::vector<Node8Q>& nodes,
+                                             std::vector<Node8>& exact) {
+  const uint32_t n = static_cast<uint32_t>(j.boxes.size());
+  std::vector<uint32_t> keys(n), ids(n), left(n - 1), right(n - 1), first(n - 1), count(n - 1),
+      parent(2 * n - 1, kLbvhNone), flag(n - 1, 0u);
+  std::vector<PrimBox> box(n - 1);
+  std::vector<Node8Q> wq(n - 1);
+  std::vector<Node8> wx(n - 1);
+  std::vector<uint2> front{make_uint2(0u, 0u)}, next;
+  uint32_t next_n = 0, wide_n = 1, maxc = 0;
+  LbvhArrays A{};
+  A.n = n;
+  A.prims = j.boxes.data();
+  A.pad = j.pad;
+  lbvh_grid(j, A.cmin, A.cscale);
+  A.keys = keys.data();
+  A.ids = ids.data();
+  A.left = left.data();
+  A.right = right.data();
+  A.first = first.data();
+  A.count = count.data();
+  A.parent = parent.data();
+  A.box = box.data();
+  A.flag = flag.data();
+  A.front_out_n = &next_n;
+  A.wide_n = &wide_n;
+  A.max_children = &maxc;
+  A.nodes = wq.data();
+  A.exact = wx.data();
+  A.node_base = node_base;
+  A.slot_base = j.slot_base;
+  for (uint32_t i = 0; i < n; ++i) lbvh_morton(A, i);
+  std::vector<uint32_t> perm(n);
+  for (uint32_t i = 0; i < n; ++i) perm[i] = i;
+  std::stable_sort(perm.begin(), perm.end(), [&](uint32_t x, uint32_t y) { return keys[x] < keys[y]; });
+  std::vector<uint32_t> sk(n);
+  for (uint32_t i = 0; i < n; ++i) sk[i] = keys[perm[i]];
+  keys.swap(sk);
+  ids = perm;
+  A.keys = keys.data();
+  A.ids = ids.data();
+  for (uint32_t i = 0; i + 1 < n; ++i) lbvh_node(A, i);
+  for (uint32_t i = 0; i < n; ++i) lbvh_bounds(A, i);
+  uint32_t stack = 0;
+  while (!front.empty()) {
+    next.assign(front.size() * CG_WIDE, make_uint2(0u, 0u));
+    next_n = 0;
+    maxc = 0;
+    A.front_in = front.data();
+    A.front_out = next.data();
+    for (const uint2& e : front) lbvh_collapse(A, e);
+    stack += maxc - 1u;
+    front.assign(next.begin(), next.begin() + next_n);
+  }
+  nodes.insert(nodes.end(), wq.begin(), wq.begin() + wide_n);
+  exact.insert(exact.end(), wx.begin(), wx.begin() + wide_n);
+  j.root = node_base;
+  j.nodes = wide_n;
+  j.stack = stack;
+  return ids;
+}
+
+// Places built BLAS: instance roots, and the stack bound (TLAS part plus the
+// deepest BLAS).
+inline void lbvh_finish(Layout& L, const std::vector<BlasJob>& jobs) {
+  uint32_t deepest = 0;
+  for (const BlasJob& j : jobs) deepest = std::max(deepest, j.stack);
+  for (InstRec& in : L.insts)
+    if ((in.root & (CG_REF_LEAF | CG_REF_INST)) == CG_REF_INST) in.root = jobs[in.root & CG_REF_INDEX_MASK].root;
+  L.stack_depth += deepest;
+}
+
+// The whole layout with every BLAS built by the linear-BVH steps on the host
+// (the device build's check; cg_scene_upload runs the same steps as kernels).
+inline int build_layout_lbvh_host(const cg_scene_data* sd, Layout& L, std::string& err) {
+  int rc = build_common(sd, L, err);
+  if (rc != CG_OK) return rc;
+  std::vector<BlasJob> jobs;
+  rc = build_sah_accel(sd, L, err, &jobs);
+  if (rc != CG_OK) return rc;
+  collapse_scene_bvh(L);
+  quantize_scene_bvh(L);
+  for (BlasJob& j : jobs) {
+    const std::vector<uint32_t> order = lbvh_build_host(j, static_cast<uint32_t>(L.bvhq.size()), L.bvhq, L.bvh);
+    std::vector<TriRec> t(order.size());
+    std::vector<uint4> v(order.size());
+    for (size_t k = 0; k < order.size(); ++k) {
+      t[k] = L.tris[j.slot_base + order[k]];
+      v[k] = L.vidx[j.slot_base + order[k]];
+    }
+    std::copy(t.begin(), t.end(), L.tris.begin() + j.slot_base);
+    std::copy(v.begin(), v.end(), L.vidx.begin() + j.slot_base);
+  }
+  lbvh_finish(L, jobs);
+  build_emitter_bvh(L);
+  return CG_OK;
+}
+
 // accel 0: the reference's trees; accel 1: rebuilt (SAH, merged world BLAS).
 // Either is traversed as an 8-wide BVH of quantised nodes.
 inline int build_layout(const cg_scene_data* sd, Layout& L, std::string& err, int accel = 1) {
+  if (accel == 2) return build_layout_lbvh_host(sd, L, err);
   int rc = build_common(sd, L, err);
   if (rc != CG_OK) return rc;
   rc = accel == 0 ? build_reference_accel(sd, L, err) : build_sah_accel(sd, L, err);

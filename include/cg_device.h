@@ -77,9 +77,12 @@ int cg_stats_read(cg_env* env, cg_stats* out);  /* synchronises */
  *   CG_ACCEL_REFERENCE: the SceneData's own BLAS/TLAS trees, node for node;
  *   CG_ACCEL_REBUILT (default): every distinct BLAS rebuilt with binned SAH
  *     (leaves of up to 4 triangles), identity-transform instances of at most 16
- *     triangles merged into one world-space BLAS, and a SAH TLAS.
- * Both give the same closest hits (up to triangles tied at the same t). */
-enum { CG_ACCEL_REFERENCE = 0, CG_ACCEL_REBUILT = 1 };
+ *     triangles merged into one world-space BLAS, and a SAH TLAS;
+ *   CG_ACCEL_DEVICE: the same instances and TLAS, every BLAS built on the GPU
+ *     as a linear BVH over Morton codes (no counterpart in the reference; for
+ *     callers that upload large or changing meshes).
+ * All give the same closest hits (up to triangles tied at the same t). */
+enum { CG_ACCEL_REFERENCE = 0, CG_ACCEL_REBUILT = 1, CG_ACCEL_DEVICE = 2 };
 int cg_env_set_accel(cg_env* env, int mode);
 int cg_stats_reset(cg_env* env);
 

@@ -159,15 +159,18 @@ def test_ray_batch_halves(cg):
 
 
 @pytest.mark.parametrize("name,params,accel", [("c0", [], 1), ("c0", [], 0), ("c1", [0.25], 1), ("spheres", [8], 1),
-                                               ("c2", [200000], 1)])
+                                               ("c2", [200000], 1), ("c0", [], 2), ("c1", [0.25], 2),
+                                               ("spheres", [8], 2), ("c2", [200000], 2), ("quad", [], 2)])
 def test_quantised_nodes_contain_exact_boxes(cg, name, params, accel):
     """The device layout's 128-B quantised 8-wide nodes decode (with the kernels' own
     float operations) to boxes that contain the exact child boxes, so the
-    traversal visits a superset of the exact tree's nodes."""
+    traversal visits a superset of the exact tree's nodes; every triangle
+    slot sits in exactly one leaf.  Accel 2 runs the device linear-BVH build's
+    per-thread steps on the host."""
     from crystalgraphics_amd import native
     s = cg.HostScene.generate(name, seed=2, params=params)
     sd = s.scene_data()
     n, depth = C.c_uint64(), C.c_uint32()
     rc = native.device().cg_layout_check(C.byref(sd.c), accel, C.byref(n), C.byref(depth))
     assert rc == 0, native.device().cg_last_error()
-    assert n.value > 0 and 1 <= depth.value <= 128
+    assert (n.value > 0 or name == "quad") and 1 <= depth.value <= 128  # quad: one instance, one leaf
