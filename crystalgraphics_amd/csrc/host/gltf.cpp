@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <functional>
 #include <limits>
@@ -42,13 +43,14 @@ namespace {
 using json::Value;
 
 bool read_file(const std::string& path, std::vector<uint8_t>& out) {
-  std::ifstream f(path, std::ios::binary | std::ios::ate);
+  std::error_code ec;
+  if (!std::filesystem::is_regular_file(path, ec)) return false;
+  const uintmax_t n = std::filesystem::file_size(path, ec);
+  if (ec || n > (uintmax_t{1} << 34)) return false;
+  std::ifstream f(path, std::ios::binary);
   if (!f) return false;
-  const std::streamoff n = f.tellg();
-  if (n < 0) return false;
   out.resize(static_cast<size_t>(n));
-  f.seekg(0);
-  if (n > 0) f.read(reinterpret_cast<char*>(out.data()), n);
+  if (n > 0) f.read(reinterpret_cast<char*>(out.data()), static_cast<std::streamsize>(n));
   return static_cast<bool>(f);
 }
 
@@ -152,7 +154,8 @@ bool decode_png(const std::vector<uint8_t>& f, HostTexture& t, std::string& err)
     }
     p += 12 + static_cast<size_t>(len);
   }
-  if (w == 0 || h == 0 || w > 65536 || h > 65536) return err = "bad PNG size", false;
+  if (w == 0 || h == 0 || w > 16384 || h > 16384 || static_cast<uint64_t>(w) * h > (uint64_t{1} << 26))
+    return err = "bad PNG size (at most 16384 on a side, 2^26 texels)", false;
   if (interlace != 0) return err = "interlaced PNG images are not supported", false;
   int ch;
   switch (ctype) {
@@ -240,7 +243,13 @@ struct Doc {
   std::string base;
 };
 
-int as_int(const Value* v, int dflt) { return v && v->is_num() ? static_cast<int>(v->num) : dflt; }
+// Integers from the file, clamped to [-1, 2^31 - 1] (negative means absent or invalid).
+int as_int(const Value* v, int dflt) {
+  if (!v || !v->is_num()) return dflt;
+  const double x = v->num;
+  if (!(x >= 0.0)) return -1;
+  return x >= 2147483647.0 ? 2147483647 : static_cast<int>(x);
+}
 float as_float(const Value* v, float dflt) { return v && v->is_num() ? static_cast<float>(v->num) : dflt; }
 const Value* arr_at(const Value& root, const char* key, int i) {
   const Value* a = root.get(key);
@@ -253,10 +262,13 @@ bool view_bytes(const Doc& d, int view, const uint8_t*& ptr, size_t& len, size_t
   if (!bv) return err = "bad bufferView index", false;
   const int b = as_int(bv->get("buffer"), -1);
   if (b < 0 || static_cast<size_t>(b) >= d.buffers.size()) return err = "bad buffer index", false;
-  const size_t off = static_cast<size_t>(as_int(bv->get("byteOffset"), 0));
-  len = static_cast<size_t>(as_int(bv->get("byteLength"), 0));
-  stride = static_cast<size_t>(as_int(bv->get("byteStride"), 0));
-  if (off + len > d.buffers[b].size()) return err = "bufferView past its buffer", false;
+  const int o = as_int(bv->get("byteOffset"), 0), l = as_int(bv->get("byteLength"), 0),
+            st = as_int(bv->get("byteStride"), 0);
+  if (o < 0 || l < 0 || st < 0 || st > 252) return err = "bad bufferView", false;
+  const size_t off = static_cast<size_t>(o);
+  len = static_cast<size_t>(l);
+  stride = static_cast<size_t>(st);
+  if (off > d.buffers[b].size() || len > d.buffers[b].size() - off) return err = "bufferView past its buffer", false;
   ptr = d.buffers[b].data() + off;
   return true;
 }
@@ -273,7 +285,11 @@ bool accessor(const Doc& d, int index, Accessor& a, std::string& err) {
   if (!v) return err = "bad accessor index", false;
   if (v->get("sparse")) return err = "sparse accessors are not supported", false;
   a.comp = as_int(v->get("componentType"), 5126);
-  a.count = static_cast<size_t>(as_int(v->get("count"), 0));
+  if (a.comp != 5120 && a.comp != 5121 && a.comp != 5122 && a.comp != 5123 && a.comp != 5125 && a.comp != 5126)
+    return err = "bad accessor componentType", false;
+  const int cnt = as_int(v->get("count"), 0);
+  if (cnt < 0 || cnt > (1 << 28)) return err = "bad accessor count", false;
+  a.count = static_cast<size_t>(cnt);
   a.normalized = v->get("normalized") && v->get("normalized")->b;
   const Value* type = v->get("type");
   const std::string t = type ? type->str : "SCALAR";
@@ -283,9 +299,12 @@ bool accessor(const Doc& d, int index, Accessor& a, std::string& err) {
     const uint8_t* p;
     size_t len, stride;
     if (!view_bytes(d, as_int(bvi, -1), p, len, stride, err)) return false;
-    const size_t off = static_cast<size_t>(as_int(v->get("byteOffset"), 0));
+    const int o = as_int(v->get("byteOffset"), 0);
+    if (o < 0) return err = "bad accessor byteOffset", false;
+    const size_t off = static_cast<size_t>(o);
     a.stride = stride ? stride : elem;
-    if (a.count && off + (a.count - 1) * a.stride + elem > len) return err = "accessor past its bufferView", false;
+    if (a.count && (off > len || (a.count - 1) * a.stride + elem > len - off))
+      return err = "accessor past its bufferView", false;
     a.data = p + off;
   } else {
     a.stride = elem;
@@ -498,7 +517,20 @@ bool has_environment(const Value& root, const Value& scene) {
 
 }  // namespace
 
+namespace {
+std::string load_gltf(const std::string& path, Scene::Impl& s);
+}  // namespace
+
 std::string LoadGltf(const std::string& path, Scene::Impl& s) {
+  try {
+    return load_gltf(path, s);
+  } catch (const std::exception& e) {
+    return std::string("cannot load: ") + e.what();
+  }
+}
+
+namespace {
+std::string load_gltf(const std::string& path, Scene::Impl& s) {
   std::vector<uint8_t> file;
   if (!read_file(path, file)) return "cannot read " + path;
   Doc d;
@@ -631,6 +663,7 @@ std::string LoadGltf(const std::string& path, Scene::Impl& s) {
           p.index_count = static_cast<uint32_t>(idx.count);
         }
         const Value* mat = pr.get("material");
+        if (mat && as_int(mat, -1) < 0) return "bad primitive material";
         p.material_idx = mat ? static_cast<uint32_t>(as_int(mat, -1) + 1) : 0u;
         if (p.material_idx >= s.materials.size()) return "primitive material index out of range";
         mesh_prims.back().push_back(p);
@@ -646,9 +679,9 @@ std::string LoadGltf(const std::string& path, Scene::Impl& s) {
   root.m[10] = 0.0;
   const Value* nodes = d.root.get("nodes");
   int visited = 0;
-  std::function<bool(int, const M4d&)> walk = [&](int ni, const M4d& parent) -> bool {
-    const Value* n = nodes ? nodes->at(static_cast<size_t>(ni)) : nullptr;
-    if (!n || ++visited > 1000000) return err = "bad node index or node cycle", false;
+  std::function<bool(int, const M4d&, int)> walk = [&](int ni, const M4d& parent, int depth) -> bool {
+    const Value* n = (nodes && ni >= 0) ? nodes->at(static_cast<size_t>(ni)) : nullptr;
+    if (!n || ++visited > 1000000 || depth > 256) return err = "bad node index, node cycle or nesting too deep", false;
     const M4d world = mul(parent, node_local(*n));
     if (const Value* mesh = n->get("mesh")) {
       const int mi = as_int(mesh, -1);
@@ -664,12 +697,12 @@ std::string LoadGltf(const std::string& path, Scene::Impl& s) {
     }
     if (const Value* ch = n->get("children"))
       for (size_t i = 0; i < ch->size(); ++i)
-        if (!walk(as_int(ch->at(i), -1), world)) return false;
+        if (!walk(as_int(ch->at(i), -1), world, depth + 1)) return false;
     return true;
   };
   if (const Value* sn = scene->get("nodes"))
     for (size_t i = 0; i < sn->size(); ++i)
-      if (!walk(as_int(sn->at(i), -1), root)) return err;
+      if (!walk(as_int(sn->at(i), -1), root, 0)) return err;
 
   // a suggested view: the scene's bounds from -y, slightly above
   float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
@@ -699,5 +732,6 @@ std::string LoadGltf(const std::string& path, Scene::Impl& s) {
   s.tag = std::hash<std::string>{}(path) ^ (static_cast<uint64_t>(file.size()) * 0x9E3779B97F4A7C15ull);
   return "";
 }
+}  // namespace
 
 }  // namespace crystal::graphics

@@ -99,10 +99,24 @@ def _variant(tmp_path, mutate):
     (lambda d: d["meshes"][1]["primitives"][0].__setitem__("material", 7), "material"),
     (lambda d: d["nodes"][2].__setitem__("children", [9]), "node"),
     (lambda d: d.__setitem__("scenes", []), "no scene"),
+    (lambda d: d["buffers"][0].__setitem__("uri", "."), "cannot read"),
+    (lambda d: d["bufferViews"][0].__setitem__("byteOffset", -8), "bad bufferView"),
+    (lambda d: d["accessors"][0].__setitem__("count", 2 ** 40), "bad accessor count"),
+    (lambda d: d["accessors"][1].__setitem__("componentType", 1234), "componentType"),
+    (lambda d: d["meshes"][0]["primitives"][0].__setitem__("material", -1), "bad primitive material"),
 ])
 def test_error_paths(cg, tmp_path, mutate, msg):
     with pytest.raises(cg.NativeError, match=msg):
         cg.LoadScene(_variant(tmp_path, mutate))
+
+
+def test_truncated_glb(cg, files, tmp_path):
+    data = files["glb"].read_bytes()
+    for cut in (10, 20, len(data) // 2, len(data) - 3):
+        p = tmp_path / f"cut{cut}.glb"
+        p.write_bytes(data[:cut])
+        with pytest.raises(cg.NativeError):
+            cg.LoadScene(str(p))
 
 
 def test_malformed_and_missing_files(cg, tmp_path):
