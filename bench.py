@@ -264,28 +264,16 @@ def batched_form(cg, seed: int, n_jobs: int, budget_gib: float, log) -> dict:
     }
 
 
-def serving_form(cg, seed: int, rate: float, seconds: float, n_scenes: int, scene_scale: float, budget_gib: float,
-                 check: int, log, texture_size: int = 2048) -> dict:
-    """The serving form: n_scenes resident scenes (16-sphere generator at
-    `scene_scale`, each with four texture_size^2 RGBA8 material textures:
-    base colour, metallic-roughness, transmission, emission), Poisson arrivals at `rate` per second for `seconds`; each
-    request: a scene, a camera on the orbit sphere, a resolution from
-    {256x256, 640x360, 1280x720, 1920x1080} weighted 4:3:2:1, spp in
-    {1, 4, 16}, depth 8.  Requests are admitted within the HBM budget and
-    batched as they arrive; latency is arrival to result.  A sample of
-    `check` requests is rendered again alone with the plain kernel and
-    compared (relative MSE of the XYZ image)."""
+def serve_run(srv, reqs: list, check: int) -> dict:
+    """Feed `reqs` ((arrival s, JobSpec)) to the server on their Poisson
+    schedule; latency is arrival to result.  A sample of `check` requests is
+    rendered again alone with the plain kernel and compared (relative MSE of
+    the XYZ image)."""
     from crystalgraphics_amd import workloads
-    t0 = time.perf_counter()
-    params = [scene_scale, 1, texture_size] if texture_size > 0 else [scene_scale]
-    hosts = [cg.HostScene.generate("c1", seed=21 + i, params=params) for i in range(n_scenes)]
-    tris = workloads.scene_triangles(scene_scale) * n_scenes
-    reqs = workloads.serve_requests(rate, seconds, n_scenes, seed=seed)
-    srv = cg.Server(hosts, budget_bytes=int(budget_gib * (1 << 30)), max_trace_depth=8)
-    setup = time.perf_counter() - t0
-    log(f"serving: {len(reqs)} requests over {seconds:.0f} s at {rate:.0f}/s, setup {setup:.1f} s")
+    rays0 = srv.stats()["rays"]
+    busy0 = srv.stats()["gpu_busy_s"]
     ids, lat, queued, kept = {}, {}, {}, {}
-    check_set = set(list(range(len(reqs)))[:: max(1, len(reqs) // max(1, check))][:check])
+    check_set = set(list(range(len(reqs)))[:: max(1, len(reqs) // max(1, check))][:check]) if check else set()
     i, t_start = 0, time.perf_counter()
     t_last = t_start
     while len(lat) < len(reqs):
@@ -304,27 +292,71 @@ def serving_form(cg, seed: int, rate: float, seconds: float, n_scenes: int, scen
             t_last = time.perf_counter()
     span = t_last - t_start
     st = srv.stats()
+    rays = st["rays"] - rays0
     rmse = []  # each checked request as served in the mix vs rendered alone with the plain kernel
     for k in sorted(kept):
         spec = reqs[k][1]
         a = srv.render_alone(spec.scene, spec.camera(), spec.conf())["history"]["xyz"].astype(np.float64)
         rmse.append(float(((kept[k] - a) ** 2).sum() / max((a ** 2).sum(), 1e-30)))
-    srv.close()
     pct = workloads.percentile
     lv = list(lat.values())
-    return {
-        "workload": f"{n_scenes} resident scenes, {tris / 1e6:.1f}M tris total, "
-                    f"{4 if texture_size > 0 else 0} x {texture_size}^2 RGBA8 textures per scene; Poisson {rate:.0f} req/s for "
-                    f"{seconds:.0f} s; 256x256/640x360/1280x720/1920x1080 weighted 4:3:2:1, spp {{1,4,16}}, depth 8",
-        "requests": len(reqs), "setup_s": round(setup, 1), "budget_gib": budget_gib,
+    out = {
+        "requests": len(reqs),
         "latency_ms": {"p50": round(pct(lv, 50) * 1e3, 1), "p99": round(pct(lv, 99) * 1e3, 1),
                        "max": round(max(lv) * 1e3, 1)},
         "queued_ms": {"p50": round(pct(list(queued.values()), 50) * 1e3, 1),
                       "p99": round(pct(list(queued.values()), 99) * 1e3, 1)},
-        "rays": st["rays"], "Mrays/s": round(st["rays"] / span / 1e6, 1), "span_s": round(span, 2),
-        "gpu_busy_frac": round(st["gpu_busy_s"] / span, 3),
-        "rmse_vs_plain": {"checked": len(rmse), "max": max(rmse) if rmse else None},
+        "rays": rays, "Mrays/s": round(rays / span / 1e6, 1), "span_s": round(span, 2),
+        "gpu_busy_frac": round((st["gpu_busy_s"] - busy0) / span, 3),
     }
+    if check:
+        out["rmse_vs_plain"] = {"checked": len(rmse), "max": max(rmse) if rmse else None}
+    return out
+
+
+def serving_form(cg, seed: int, rate: float, seconds: float, n_scenes: int, scene_scale: float, budget_gib: float,
+                 check: int, log, texture_size: int = 2048, sustainable: float = 0.8) -> dict:
+    """The serving form: n_scenes resident scenes (16-sphere generator at
+    `scene_scale`, each with four texture_size^2 RGBA8 material textures:
+    base colour, metallic-roughness, transmission, emission), Poisson
+    arrivals at `rate` per second for `seconds`; each request: a scene, a
+    camera on the orbit sphere, a resolution from {256x256, 640x360,
+    1280x720, 1920x1080} weighted 4:3:2:1, spp in {1, 4, 16}, depth 8.
+    Requests are admitted within the HBM budget and batched as they arrive.
+    The capacity in requests per second follows from the served rate and the
+    mean queries per request; when `rate` exceeds it, requests queue and
+    latency grows with the run's length, so a second run at `sustainable`
+    times the capacity reports the latency of a load the GPU keeps up with."""
+    from crystalgraphics_amd import workloads
+    t0 = time.perf_counter()
+    params = [scene_scale, 1, texture_size] if texture_size > 0 else [scene_scale]
+    hosts = [cg.HostScene.generate("c1", seed=21 + i, params=params) for i in range(n_scenes)]
+    tris = workloads.scene_triangles(scene_scale) * n_scenes
+    srv = cg.Server(hosts, budget_bytes=int(budget_gib * (1 << 30)), max_trace_depth=8)
+    setup = time.perf_counter() - t0
+    reqs = workloads.serve_requests(rate, seconds, n_scenes, seed=seed)
+    log(f"serving: {len(reqs)} requests over {seconds:.0f} s at {rate:.0f}/s, setup {setup:.1f} s")
+    out = {
+        "workload": f"{n_scenes} resident scenes, {tris / 1e6:.1f}M tris total, "
+                    f"{4 if texture_size > 0 else 0} x {texture_size}^2 RGBA8 textures per scene; Poisson {rate:.0f} req/s for "
+                    f"{seconds:.0f} s; 256x256/640x360/1280x720/1920x1080 weighted 4:3:2:1, spp {{1,4,16}}, depth 8",
+        "setup_s": round(setup, 1), "budget_gib": budget_gib, "rate_req_per_s": rate,
+    }
+    out.update(serve_run(srv, reqs, check))
+    per_req = out["rays"] / max(1, out["requests"])
+    cap = out["Mrays/s"] * 1e6 / per_req
+    out["queries_per_request"] = round(per_req / 1e6, 2)
+    out["offered_Mrays_per_s"] = round(rate * per_req / 1e6, 1)
+    out["capacity_req_per_s"] = round(cap, 2)
+    if sustainable > 0:
+        r2 = sustainable * cap
+        reqs2 = workloads.serve_requests(r2, seconds, n_scenes, seed=seed + 1)
+        log(f"serving: {len(reqs2)} requests at {r2:.1f}/s ({sustainable:.0%} of capacity)")
+        sub = serve_run(srv, reqs2, 0)
+        sub["rate_req_per_s"] = round(r2, 2)
+        out["at_sustainable_rate"] = sub
+    srv.close()
+    return out
 
 
 def main() -> int:
