@@ -241,7 +241,7 @@ CG_HD void cswap(float& ta, uint32_t& ra, float& tb, uint32_t& rb) {
   }
 }
 
-// Two-level closest-hit (or any-hit) traversal of the 4-wide BVH: the hit
+// Two-level closest-hit (or any-hit) traversal of the 8-wide BVH: the hit
 // children of a node sorted by entry distance, the nearest visited next and
 // the others pushed with their distances; popped entries entered beyond the
 // best hit so far are dropped; a marker on the stack returns from a BLAS to
@@ -285,7 +285,7 @@ CG_HD bool trav_begin(const TravScene& S, TravRay& r, f3 wo, f3 wd, float tmax, 
   return r.ref != CG_REF_NONE;
 }
 
-// Two-level closest-hit (or any-hit) traversal of the 4-wide BVH, one visit
+// Two-level closest-hit (or any-hit) traversal of the 8-wide BVH, one visit
 // per call: at a node the hit children are sorted by entry distance, the
 // nearest is visited next and the others are pushed with their distances;
 // popped entries entered beyond the best hit so far are dropped; a marker on

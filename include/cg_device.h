@@ -42,7 +42,7 @@ typedef struct cg_scene cg_scene;
 
 /* Traversal counters accumulated by cg_view / cg_trace_* while counting is
  * enabled (and always by a batch): closest-hit and any-hit queries, 128-B
- * 4-wide BVH nodes visited, 48-B triangle records tested, 80-B instance
+ * 8-wide BVH nodes visited, 48-B triangle records tested, 80-B instance
  * records entered. */
 typedef struct cg_stats {
   uint64_t rays;
