@@ -114,15 +114,33 @@ int dev_upload(cg_env* env, DevBuf& b, const void* src, size_t bytes) {
 // exhausted, steals from the next XCD's.  The grid is the resident size, so
 // waves loop until every range is empty; waves never wait for each other.
 constexpr uint32_t kXcds = 8u;
+#ifndef CG_VB_LPT
+#define CG_VB_LPT 1
+#endif
 constexpr uint32_t kNoItem = 0xFFFFFFFFu;
 constexpr uint32_t kCounterStride = 32u;  // one 128-byte line per counter
 struct WorkQueue {
   uint32_t* counters;  // kXcds counters, kCounterStride apart, zero at launch
   uint32_t total;      // items
+  uint32_t interleave; // 0: XCD x owns a contiguous range; 1: XCD x owns items x, x + 8, x + 16, ...
 };
 __device__ __forceinline__ uint32_t grab_item(const WorkQueue& q, uint32_t& probe) {
   uint32_t item = kNoItem;
-  if ((threadIdx.x & 63u) == 0u) {
+  if ((threadIdx.x & 63u) == 0u && q.interleave) {
+    const uint32_t home = blockIdx.x % kXcds;
+    while (probe < kXcds) {
+      const uint32_t x = (home + probe) % kXcds;
+      uint32_t* ctr = q.counters + x * kCounterStride;
+      if (x + uint64_t{kXcds} * __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < q.total) {
+        const uint64_t i = x + uint64_t{kXcds} * atomicAdd(ctr, 1u);
+        if (i < q.total) {
+          item = static_cast<uint32_t>(i);
+          break;
+        }
+      }
+      ++probe;
+    }
+  } else if ((threadIdx.x & 63u) == 0u) {
     const uint32_t per = (q.total + kXcds - 1u) / kXcds;
     const uint32_t home = blockIdx.x % kXcds;
     while (probe < kXcds) {
@@ -1165,7 +1183,19 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
   std::vector<ViewArgs> va(n);
   std::vector<uint32_t> first(n + 1, 0u);
   uint32_t depth = 1, frames = 1;
-  for (uint32_t i = 0; i < n; ++i) {
+  // Tiles in order of their expected cost (samples x trace depth per pixel),
+  // dealt round-robin to the XCDs: every XCD starts on its most expensive
+  // tiles and the launch ends on cheap ones instead of a few long tiles.
+  std::vector<uint32_t> ord(n);
+  for (uint32_t i = 0; i < n; ++i) ord[i] = i;
+#if CG_VB_LPT
+  auto cost = [&](uint32_t i) {
+    return static_cast<uint64_t>(jobs[i].conf.sample_count) * std::min(std::max(jobs[i].conf.trace_depth, 1u), 64u);
+  };
+  std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
+#endif
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t i = ord[k];
     const cg_job* job = jobs + i;
     if (!job->scene || !job->history) return fail(env, CG_ERR_INVALID, "job %u: null scene or history", i);
     if (std::find(env->scenes.begin(), env->scenes.end(), job->scene) == env->scenes.end())
@@ -1173,7 +1203,7 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
     const cg_conf& c = job->conf;
     if (c.render_width == 0 || c.render_height == 0) return fail(env, CG_ERR_INVALID, "job %u: empty render size", i);
     if (job->path_hash && !job->ray_count) return fail(env, CG_ERR_INVALID, "job %u: path_hash needs ray_count", i);
-    ViewArgs& a = va[i];
+    ViewArgs& a = va[k];
     a = ViewArgs{};
     a.S = job->scene->view;
     a.P.D = static_cast<int>(std::min(std::max(c.trace_depth, 1u), 64u));
@@ -1200,8 +1230,8 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
     a.ray_count = job->path_hash ? job->ray_count : nullptr;
     a.stats = nullptr;
     const uint64_t tiles = static_cast<uint64_t>((c.render_width + 7u) / 8u) * ((c.render_height + 7u) / 8u);
-    if (first[i] + tiles > 0xFFFFFF00ull) return fail(env, CG_ERR_INVALID, "batch too large");
-    first[i + 1] = first[i] + static_cast<uint32_t>(tiles);
+    if (first[k] + tiles > 0x7FFFFF00ull) return fail(env, CG_ERR_INVALID, "batch too large");
+    first[k + 1] = first[k] + static_cast<uint32_t>(tiles);
     depth = std::max(depth, job->scene->stack_depth);
     frames = std::max(frames, static_cast<uint32_t>(std::max(1, a.P.D - 1)));
   }
@@ -1248,7 +1278,7 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
     return give_back(fail(env, CG_ERR_HIP, "cg_view_batch_launch: job table copy failed"));
   uint32_t* ctr = static_cast<uint32_t*>(v->queue.p);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kXcds * kCounterStride), 0, v->stream, ctr, kXcds * kCounterStride);
-  const WorkQueue q{ctr, items};
+  const WorkQueue q{ctr, items, CG_VB_LPT ? 1u : 0u};
   unsigned long long* stats = env->counting ? static_cast<unsigned long long*>(env->stats.p) : nullptr;
   if (frames <= 4)
     launch_view_batch_mf<4>(v, grid, n, q, sa, stats);
