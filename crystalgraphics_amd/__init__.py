@@ -563,6 +563,22 @@ def LoadScene(path: str) -> HostScene:
     return HostScene(h.value)
 
 
+def SubmitJobs(scenes: list, requests: list) -> tuple[list[np.ndarray], float]:
+    """Batch form through the EnvInit environment: requests are (scene index, Camera, Conf); every request
+    rendered in one launch (a new accumulation each).  Returns the (H, W, 3) XYZ images in request order and
+    the batch's wall time in seconds."""
+    n = len(requests)
+    sc = (C.c_void_p * max(1, len(scenes)))(*[s.handle for s in scenes])
+    idx = np.array([r[0] for r in requests], np.uint32)
+    cams = (Camera * max(1, n))(*[r[1] for r in requests])
+    confs = (Conf * max(1, n))(*[r[2] for r in requests])
+    outs = [np.zeros((r[2].render_height, r[2].render_width, 3), np.float32) for r in requests]
+    ptrs = (C.c_void_p * max(1, n))(*[o.ctypes.data for o in outs])
+    secs = C.c_double()
+    _host_check(native.host().cgh_submit_jobs(sc, len(scenes), idx.ctypes.data, cams, confs, n, ptrs, C.byref(secs)))
+    return outs, secs.value
+
+
 def View(scene: HostScene, camera: Camera | None = None) -> None:
     _host_check(native.host().cgh_view(scene.handle, C.byref(camera or Camera.make((0, 0, 0), (1, 0, 0)))))
 

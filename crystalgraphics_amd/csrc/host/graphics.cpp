@@ -8,6 +8,7 @@
 // a scene once per tag as the reference's CacheTag does
 // (src/wgpu/view.cpp:25-55).  FrameHistory (src/wgpu/frame_history.h:12-47)
 // restarts accumulation when the scene or any camera float changes.
+#include <chrono>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -177,6 +178,86 @@ expected<void, Error> ViewToBuffer(const Scene& scene, const Camera& camera, flo
     return unexpected<Error>(Error{cg_env_last_error(g_env->backend)});
   for (size_t i = 0; i < px; ++i) std::memcpy(xyz + 3 * i, h[i].xyz, 3 * sizeof(float));
   return {};
+}
+
+expected<std::vector<RenderResult>, Error> SubmitJobs(const std::vector<const Scene*>& scenes,
+                                                      const std::vector<RenderRequest>& requests) {
+  if (!g_env) return unexpected<Error>(Error{"Environment not initialized."});
+  Env& e = *g_env;
+  std::vector<RenderResult> out(requests.size());
+  if (requests.empty()) return out;
+  std::map<size_t, cg_scene*> uploaded;  // scene index -> device scene (released at the end unless it is e.uploaded)
+  std::vector<void*> buffers;
+  auto cleanup = [&] {
+    for (void* b : buffers) cg_device_free(e.backend, b);
+    for (auto& kv : uploaded)
+      if (kv.second != e.uploaded) cg_scene_release(e.backend, kv.second);
+  };
+  auto fail = [&](const std::string& msg) {
+    cleanup();
+    return unexpected<Error>(Error{msg});
+  };
+  std::vector<cg_job> jobs(requests.size());
+  for (size_t k = 0; k < requests.size(); ++k) {
+    const RenderRequest& r = requests[k];
+    if (r.scene >= scenes.size() || !scenes[r.scene]) return fail("SubmitJobs: request " + std::to_string(k) + " names no scene");
+    const PathTraceConf& c = r.conf;
+    if (c.render_width == 0 || c.render_height == 0) return fail("SubmitJobs: request " + std::to_string(k) + " has an empty size");
+    if (!uploaded.count(r.scene)) {
+      const Scene::Impl& impl = scenes[r.scene]->impl();
+      if (e.uploaded && e.uploaded_tag == impl.tag) {
+        uploaded[r.scene] = e.uploaded;
+      } else {
+        auto it = e.scene_data.find(impl.tag);
+        if (it == e.scene_data.end())
+          it = e.scene_data.emplace(impl.tag, std::make_unique<SceneData>(BuildSceneData(impl, true, default_threads())))
+                   .first;
+        cg_scene_data view = it->second->View();
+        cg_scene* ds = nullptr;
+        if (cg_scene_upload(e.backend, &view, &ds) != CG_OK) return fail(cg_env_last_error(e.backend));
+        uploaded[r.scene] = ds;
+      }
+    }
+    const size_t px = static_cast<size_t>(c.render_width) * c.render_height;
+    void* h = nullptr;
+    if (cg_device_alloc(e.backend, px * sizeof(cg_history_pixel), &h) != CG_OK) return fail(cg_env_last_error(e.backend));
+    buffers.push_back(h);
+    cg_job& j = jobs[k];
+    j = cg_job{};
+    j.scene = uploaded[r.scene];
+    j.camera = to_cg(r.camera);
+    j.conf = cg_conf{c.window_width, c.window_height, c.render_width, c.render_height, c.sample_count, c.trace_depth,
+                     c.max_transport_sample_count, c.max_diffuse_sample_count, c.max_rough_sample_count,
+                     c.max_emission_sample_count};
+    j.iter_count = 0;
+    j.history = static_cast<cg_history_pixel*>(h);
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  cg_vbatch* vb = nullptr;
+  if (cg_view_batch_launch(e.backend, jobs.data(), static_cast<uint32_t>(jobs.size()), &vb) != CG_OK)
+    return fail(cg_env_last_error(e.backend));
+  if (cg_view_batch_release(vb) != CG_OK) return fail(cg_env_last_error(e.backend));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (size_t k = 0; k < requests.size(); ++k) {
+    const uint32_t w = requests[k].conf.render_width, hgt = requests[k].conf.render_height;
+    std::vector<cg_history_pixel> hist(static_cast<size_t>(w) * hgt);
+    if (cg_memcpy_d2h(e.backend, hist.data(), jobs[k].history, hist.size() * sizeof(cg_history_pixel)) != CG_OK)
+      return fail(cg_env_last_error(e.backend));
+    RenderResult& o = out[k];
+    o.id = k + 1;
+    o.width = w;
+    o.height = hgt;
+    o.xyz.resize(hist.size() * 3);
+    o.seeds.resize(hist.size());
+    for (size_t i = 0; i < hist.size(); ++i) {
+      std::memcpy(&o.xyz[3 * i], hist[i].xyz, 3 * sizeof(float));
+      o.seeds[i] = hist[i].rand_seed;
+    }
+    o.queued_s = 0.0;
+    o.latency_s = secs;
+  }
+  cleanup();
+  return out;
 }
 
 }  // namespace crystal::graphics
@@ -375,6 +456,40 @@ int cgh_view(const cgh_scene* s, const cg_camera* cam) {
 int cgh_view_to_buffer(const cgh_scene* s, const cg_camera* cam, float* xyz, size_t count) {
   return wrap(ViewToBuffer(*reinterpret_cast<const Scene*>(s), from_cg(cam), xyz, count));
 }
+// SubmitJobs for bindings: requests as (scene index, camera, conf) arrays;
+// xyz[k] receives W*H*3 floats of request k.
+int cgh_submit_jobs(const cgh_scene* const* scenes, uint32_t nscenes, const uint32_t* scene_of, const cg_camera* cams,
+                    const cg_conf* confs, uint32_t n, float* const* xyz, double* seconds) {
+  std::vector<const Scene*> sc(nscenes);
+  for (uint32_t i = 0; i < nscenes; ++i) sc[i] = reinterpret_cast<const Scene*>(scenes[i]);
+  std::vector<RenderRequest> reqs(n);
+  for (uint32_t k = 0; k < n; ++k) {
+    reqs[k].scene = scene_of[k];
+    reqs[k].camera = from_cg(&cams[k]);
+    const cg_conf& c = confs[k];
+    PathTraceConf p;
+    p.window_width = c.window_width;
+    p.window_height = c.window_height;
+    p.render_width = c.render_width;
+    p.render_height = c.render_height;
+    p.sample_count = c.sample_count;
+    p.trace_depth = c.trace_depth;
+    p.max_transport_sample_count = c.max_transport_sample_count;
+    p.max_diffuse_sample_count = c.max_diffuse_sample_count;
+    p.max_rough_sample_count = c.max_rough_sample_count;
+    p.max_emission_sample_count = c.max_emission_sample_count;
+    reqs[k].conf = p;
+  }
+  auto r = SubmitJobs(sc, reqs);
+  if (!r) {
+    t_err = r.error();
+    return CG_ERR_STATE;
+  }
+  for (uint32_t k = 0; k < n; ++k) std::memcpy(xyz[k], (*r)[k].xyz.data(), (*r)[k].xyz.size() * sizeof(float));
+  if (seconds) *seconds = n ? (*r)[0].latency_s : 0.0;
+  return CG_OK;
+}
+
 int cgh_load_scene(const char* path, cgh_scene** out) {
   *out = nullptr;
   auto r = LoadScene(path);

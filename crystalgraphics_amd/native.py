@@ -287,6 +287,7 @@ def host() -> C.CDLL:
         _sig(h, "cgh_scene_create", I32, P, U32, P, U32, P, U32, P, U32, P, U32, P, U32, U32, C.c_uint64,
              C.POINTER(P))
         _sig(h, "cgh_scene_free", None, P)
+        _sig(h, "cgh_submit_jobs", I32, P, U32, P, P, P, U32, P, C.POINTER(C.c_double))
         _sig(h, "cgh_scene_describe", I32, P, C.POINTER(SceneDescC))
         _sig(h, "cgh_rays_generate", I32, P, C.c_uint64, U32, P)
         _sig(h, "cgh_scene_data_build", I32, P, I32, I32, C.POINTER(P))

@@ -57,6 +57,14 @@ struct ServeStats {
   double gpu_busy_s = 0;              // time with at least one request in flight
 };
 
+// Batch form (a one-shot call; no counterpart in the reference, which renders one View at a time):
+// every request rendered in one launch of the per-pixel kernel through the
+// environment of EnvInit (each request at its own size, spp and depth, a new
+// accumulation), results in request order.  `scenes` are uploaded as needed
+// and released afterwards.  latency_s is the batch's wall time.
+expected<std::vector<RenderResult>, Error> SubmitJobs(const std::vector<const Scene*>& scenes,
+                                                      const std::vector<RenderRequest>& requests);
+
 class Server {
  public:
   // Uploads every scene (they stay resident) and sizes the slot pool and the

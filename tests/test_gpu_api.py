@@ -80,3 +80,32 @@ def test_gltf_scene_renders_like_the_oracle(cg, oracle, tmp_path):
           f"lit pixels {(np.abs(ref['history']['xyz']).max(axis=-1) > 0).mean() * 100:.1f}%")
     assert same.mean() >= 0.99 and err[same].max() <= 1e-4
     assert np.array_equal(rcount[same], ref["ray_count"][same])
+
+
+def test_submit_jobs_equal_single_renders(cg):
+    """SubmitJobs (many requests, two scenes, one launch) returns for every
+    request exactly the image of rendering it alone with the plain kernel."""
+    from crystalgraphics_amd import workloads
+    scenes = [cg.HostScene.generate("spheres", seed=4, params=[6]),
+              cg.HostScene.generate("c1", seed=3, params=[0.125, 1, 128])]
+    rng = np.random.default_rng(9)
+    reqs = []
+    for k in range(7):
+        w, h = [(32, 24), (48, 40), (17, 11), (64, 36)][k % 4]
+        pos, d = workloads.orbit_camera(rng, 3.0 if k % 2 else 12.0)
+        reqs.append((k % 2, cg.Camera.make(pos, d, (0.5 * w / h, 0.5)), cg.Conf.make(w, h, [1, 2, 4][k % 3], [2, 5, 8][k % 3])))
+    cg.EnvInit(cg.Conf.make(64, 64, 1, 2))
+    try:
+        outs, secs = cg.SubmitJobs(scenes, reqs)
+    finally:
+        cg.EnvTerm()
+    assert secs > 0
+    with cg.DeviceEnv(cg.Conf.make(8, 8)) as env:
+        sds = [s.scene_data() for s in scenes]
+        dss = [env.upload(sd) for sd in sds]
+        for (si, cam, conf), got in zip(reqs, outs):
+            rj = cg.RenderJob(env, dss[si], cam, conf)
+            rj.view_alone()
+            ref = rj.read_history()["xyz"].reshape(conf.render_height, conf.render_width, 3)
+            assert got.tobytes() == ref.tobytes()
+            rj.free()

@@ -31,7 +31,7 @@ def test_host_library_exports(cg):
     from crystalgraphics_amd import native
     h = native.host()
     for n in ("cgh_scene_generate", "cgh_scene_data_build", "cgh_env_init", "cgh_view", "cgh_view_to_buffer",
-              "cgh_load_scene", "cgh_env_term"):
+              "cgh_load_scene", "cgh_env_term", "cgh_submit_jobs", "cgh_server_create"):
         assert hasattr(h, n)
 
 
@@ -83,6 +83,8 @@ def test_api_error_paths(cg):
     s = cg.HostScene.generate("quad")
     with pytest.raises(cg.NativeError, match="Environment not initialized"):
         cg.View(s, s.describe()["camera"])
+    with pytest.raises(cg.NativeError, match="Environment not initialized"):
+        cg.SubmitJobs([s], [(0, s.describe()["camera"], cg.Conf.make(8, 8, 1, 2))])
     with pytest.raises(cg.NativeError, match="already terminated"):
         cg.EnvTerm()
     with pytest.raises(cg.NativeError, match="not supported"):
