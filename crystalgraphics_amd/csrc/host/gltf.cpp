@@ -729,7 +729,15 @@ std::string load_gltf(const std::string& path, Scene::Impl& s) {
   s.camera.viewport[0] = 0.5f * 16.0f / 9.0f;
   s.camera.viewport[1] = 0.5f;
   s.conf = cg_conf{1920, 1080, 640, 360, 4, 4, 3, 1, 1, 1};
-  s.tag = std::hash<std::string>{}(path) ^ (static_cast<uint64_t>(file.size()) * 0x9E3779B97F4A7C15ull);
+  // the tag names the content (the facade caches a scene's prepared data per tag)
+  uint64_t h = 0xCBF29CE484222325ull;
+  auto mix = [&](const uint8_t* p, size_t n) {
+    for (size_t i = 0; i < n; ++i) h = (h ^ p[i]) * 0x100000001B3ull;
+  };
+  mix(file.data(), file.size());
+  for (const auto& b : d.buffers) mix(b.data(), b.size());
+  for (const auto& t : s.textures) mix(t.rgba8.data(), t.rgba8.size());
+  s.tag = h ^ std::hash<std::string>{}(path);
   return "";
 }
 }  // namespace

@@ -126,3 +126,16 @@ def test_malformed_and_missing_files(cg, tmp_path):
         cg.LoadScene(str(bad))
     with pytest.raises(cg.NativeError, match="cannot read"):
         cg.LoadScene(str(tmp_path / "missing.glb"))
+
+
+def test_tag_follows_content(cg, tmp_path):
+    """The scene tag (the facade's cache key) changes with the file's content,
+    not only with its path and size."""
+    p = gf.write(tmp_path, "datauri")
+    t1 = cg.LoadScene(str(p)).describe()["tag"]
+    doc = json.loads(p.read_text())
+    doc["materials"][0]["pbrMetallicRoughness"]["roughnessFactor"] = 0.5  # same length as 0.6
+    p.write_text(json.dumps(doc))
+    t2 = cg.LoadScene(str(p)).describe()["tag"]
+    assert t1 != t2
+    assert cg.LoadScene(str(p)).describe()["tag"] == t2
