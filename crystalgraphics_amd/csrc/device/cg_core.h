@@ -309,15 +309,24 @@ CG_HD bool trav_step(const TravScene& S, TravRay& r, const TStack& stk, TravStat
                 sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23);
     float t[CG_WIDE];
     uint32_t c[CG_WIDE] = {w4.x, w4.y, w4.z, w4.w, w5.x, w5.y, w5.z, w5.w};
+    // The slab distance of a grid line q is (origin + q * step - o) * inv =
+    // q * (step * inv) + (origin - o) * inv: one multiply-add per bound.  Its
+    // rounding differs from the unfolded form by a few ulps, far inside the
+    // boxes' outward padding, so no box holding a hit is culled and the
+    // closest hits are unchanged.
+    const float ax = sx * r.inv.x, ay = sy * r.inv.y, az = sz * r.inv.z;
+    const float bx = (ox - r.o.x) * r.inv.x, by = (oy - r.o.y) * r.inv.y, bz = (oz - r.o.z) * r.inv.z;
 #define CG_QB(w, i) static_cast<float>(((w) >> (8 * ((i) & 3))) & 0xFFu)
 #pragma unroll
     for (int i = 0; i < CG_WIDE; ++i) {
       const uint32_t lx = i < 4 ? w1.x : w1.y, ly = i < 4 ? w1.z : w1.w, lz = i < 4 ? w2.x : w2.y;
       const uint32_t hx = i < 4 ? w2.z : w2.w, hy = i < 4 ? w3.x : w3.y, hz = i < 4 ? w3.z : w3.w;
-      const float b[3] = {ox + CG_QB(lx, i) * sx, oy + CG_QB(ly, i) * sy, oz + CG_QB(lz, i) * sz};
-      const float e[3] = {ox + CG_QB(hx, i) * sx, oy + CG_QB(hy, i) * sy, oz + CG_QB(hz, i) * sz};
-      const float tn = box_enter(r.o, r.inv, b, e);
-      t[i] = (tn <= r.best.t && c[i] != CG_REF_NONE) ? tn : kInf;
+      const float t0x = fmaf(CG_QB(lx, i), ax, bx), t1x = fmaf(CG_QB(hx, i), ax, bx);
+      const float t0y = fmaf(CG_QB(ly, i), ay, by), t1y = fmaf(CG_QB(hy, i), ay, by);
+      const float t0z = fmaf(CG_QB(lz, i), az, bz), t1z = fmaf(CG_QB(hz, i), az, bz);
+      const float tn = fmx(fmx(fmn(t0x, t1x), fmn(t0y, t1y)), fmn(t0z, t1z));
+      const float tf = fmn(fmn(fmx(t0x, t1x), fmx(t0y, t1y)), fmx(t0z, t1z));
+      t[i] = (tf >= tn && tf > 0.0f && tn <= r.best.t && c[i] != CG_REF_NONE) ? tn : kInf;
     }
 #undef CG_QB
     // nearest first: an 8-input sorting network (19 compare-exchanges)
