@@ -158,10 +158,8 @@ CG_QHD void lbvh_bounds(const LbvhArrays& A, uint32_t leaf) {
   // a Karras tree is at most 62 levels deep (30 key bits, 32 position bits)
   for (int level = 0; p != kLbvhNone && level < 64; ++level) {
     if (lbvh_arrive(&A.flag[p]) == 0u) return;
+    // the release half of the next node's arrival publishes this box
     A.box[p] = lbvh_union(lbvh_child_box(A, A.left[p]), lbvh_child_box(A, A.right[p]));
-#if defined(__HIP_DEVICE_COMPILE__)
-    __threadfence();
-#endif
     p = A.parent[p];
   }
 }
