@@ -227,8 +227,20 @@ class DeviceEnv:
 
     ACCEL_REFERENCE, ACCEL_REBUILT = 0, 1
 
+    def refit(self, scene: DeviceScene, sd) -> None:
+        """New data (same triangles and BLASes) for a scene uploaded with accel 2: BLAS boxes refitted on the
+        device, TLAS and other records rebuilt."""
+        keep = None
+        if isinstance(sd, SceneData):
+            c = sd.c
+        else:
+            c, keep = native.make_scene_data(sd)
+        self._check(native.device().cg_scene_refit(self._h, scene.handle, C.byref(c)))
+        del keep
+
     def set_accel(self, mode: int) -> None:
-        """Acceleration layout for later uploads (0: the reference's trees, 1: rebuilt)."""
+        """Acceleration layout for later uploads (0: the reference's trees, 1: rebuilt SAH, 2: every BLAS
+        built on the device)."""
         self._check(native.device().cg_env_set_accel(self._h, int(mode)))
 
     def set_counting(self, on: bool = True) -> None:

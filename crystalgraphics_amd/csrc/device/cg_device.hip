@@ -33,11 +33,22 @@ struct DevBuf {
 
 }  // namespace
 
+// A BLAS built on the device, kept for refits: its triangles' sorted order,
+// the binary tree, and each wide node's binary children.
+struct BlasState {
+  DevBuf ids, left, right, parent, kids;
+  uint32_t n = 0, slot_base = 0, node_base = 0, nodes = 0, stack = 0;
+};
+
 struct cg_scene {
   DevBuf bvh, nodes, tris, insts, vidx, vattr, mats, emit, alias, emit_order, env, tex;
   DevScene view{};
   uint32_t stack_depth = 0;
   uint64_t bytes = 0;
+  bool device_built = false;      // uploaded with CG_ACCEL_DEVICE: BLAS nodes first, then the TLAS
+  std::vector<BlasState> blas;
+  uint32_t blas_nodes = 0;        // BLAS nodes before the TLAS's
+  uint64_t bvh_capacity = 0;      // nodes the bvh buffer holds
 };
 
 struct cg_vbatch {
@@ -410,6 +421,10 @@ __global__ __launch_bounds__(kBlock) void k_lbvh_collapse(LbvhArrays A, uint32_t
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i < count) lbvh_collapse(A, A.front_in[i]);
 }
+__global__ __launch_bounds__(kBlock) void k_lbvh_refit(LbvhArrays A, uint32_t nodes) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < nodes) lbvh_refit_wide(A, i);
+}
 __global__ __launch_bounds__(kBlock) void k_lbvh_gather(const TriRec* __restrict__ tin, const uint4* __restrict__ vin,
                                                         TriRec* __restrict__ tout, uint4* __restrict__ vout,
                                                         const uint32_t* __restrict__ ids, uint32_t n) {
@@ -582,18 +597,84 @@ void release_scene(cg_scene* s) {
   dev_free(s->emit_order);
   dev_free(s->env);
   dev_free(s->tex);
+  for (BlasState& b : s->blas)
+    for (DevBuf* d : {&b.ids, &b.left, &b.right, &b.parent, &b.kids}) dev_free(*d);
+  s->blas.clear();
+}
+
+// Every record but the BVH nodes and triangles (emitter BVH, instances,
+// vertex attributes, materials, emitters, alias table, environment, textures).
+int upload_records(cg_env* env, cg_scene* s, const cg_scene_data* sd, const Layout& L, uint64_t& total) {
+  int rc = CG_OK;
+  auto up = [&](int r) {
+    if (r != CG_OK && rc == CG_OK) rc = r;
+  };
+  up(upload_vec(env, s->nodes, L.nodes, total));
+  up(upload_vec(env, s->insts, L.insts, total));
+  up(upload_vec(env, s->vattr, L.vattr, total));
+  up(upload_vec(env, s->mats, L.mats, total));
+  up(upload_vec(env, s->emit, L.emit, total));
+  up(upload_vec(env, s->alias, L.alias, total));
+  up(upload_vec(env, s->emit_order, L.emit_order, total));
+  const size_t env_bytes = static_cast<size_t>(sd->env_width) * sd->env_height * 4 * sizeof(float);
+  total += env_bytes;
+  up(dev_upload(env, s->env, sd->env_rgba32f, env_bytes));
+  const size_t tex_bytes = static_cast<size_t>(sd->tex_width) * sd->tex_height * 4 * sd->tex_layers;
+  if (tex_bytes) {
+    total += tex_bytes;
+    up(dev_upload(env, s->tex, sd->tex_rgba8, tex_bytes));
+  }
+  return rc;
+}
+// The kernels' view of an uploaded scene.
+void set_view(cg_scene* s, const cg_scene_data* sd, const Layout& L) {
+  DevScene& v = s->view;
+  v.bvh = static_cast<const Node8Q*>(s->bvh.p);
+  v.nodes = static_cast<const PairNode*>(s->nodes.p);
+  v.tris = static_cast<const TriRec*>(s->tris.p);
+  v.insts = static_cast<const InstRec*>(s->insts.p);
+  v.tri_vidx = static_cast<const uint4*>(s->vidx.p);
+  v.vattr = static_cast<const VAttr*>(s->vattr.p);
+  v.mats = static_cast<const MatRec*>(s->mats.p);
+  v.mat_count = static_cast<uint32_t>(L.mats.size());
+  v.emit = static_cast<const EmitRec*>(s->emit.p);
+  v.emit_count = static_cast<uint32_t>(L.emit.size());
+  v.alias = static_cast<const float2*>(s->alias.p);
+  v.alias_count = static_cast<uint32_t>(L.alias.size());
+  v.emit_order = static_cast<const uint32_t*>(s->emit_order.p);
+  v.emit_root = L.emit_root;
+  v.env = static_cast<const float4*>(s->env.p);
+  v.env_w = sd->env_width;
+  v.env_h = sd->env_height;
+  const bool tex = s->tex.p != nullptr;
+  v.tex = static_cast<const uint32_t*>(s->tex.p);
+  v.tex_w = tex ? sd->tex_width : 0u;
+  v.tex_h = tex ? sd->tex_height : 0u;
+  v.root = L.root;
+  s->stack_depth = L.stack_depth;
+}
+
+// The host-built TLAS nodes (refs among themselves from 0) moved to start at
+// `base`: their interior refs and the root shift; instance and leaf refs stay.
+void place_tlas(Layout& L, uint32_t base) {
+  auto shift = [base](uint32_t& r) {
+    if (r != CG_REF_NONE && (r & (CG_REF_LEAF | CG_REF_INST)) == 0u) r += base;
+  };
+  for (Node8Q& q : L.bvhq)
+    for (int k = 0; k < CG_WIDE; ++k) shift(q.ref[k]);
+  shift(L.root);
 }
 
 // Builds one BLAS with the linear-BVH kernels: sorts its triangles, writes
 // their records into tris / vidx at the job's slots (from the source-order
 // copies), and returns its wide nodes (global refs from node_base) in `nodes`.
 int lbvh_build_device(cg_env* env, BlasJob& j, uint32_t node_base, const TriRec* tris_src, const uint4* vidx_src,
-                      TriRec* tris, uint4* vidx, DevBuf& nodes) {
+                      TriRec* tris, uint4* vidx, DevBuf& nodes, BlasState& keep) {
   const uint32_t n = static_cast<uint32_t>(j.boxes.size());
-  DevBuf prims, keys, keys2, ids, ids2, left, right, first, count, parent, box, flag, fa, fb, ctr, tmp;
+  DevBuf prims, keys, keys2, ids, ids2, left, right, first, count, parent, box, flag, fa, fb, ctr, tmp, kids;
   auto release = [&] {
     for (DevBuf* b : {&prims, &keys, &keys2, &ids, &ids2, &left, &right, &first, &count, &parent, &box, &flag, &fa,
-                      &fb, &ctr, &tmp})
+                      &fb, &ctr, &tmp, &kids})
       dev_free(*b);
   };
   auto grid = [](uint32_t items) { return dim3((items + kBlock - 1u) / kBlock); };
@@ -610,6 +691,7 @@ int lbvh_build_device(cg_env* env, BlasJob& j, uint32_t node_base, const TriRec*
   alloc(fb, (n - 1u) * sizeof(uint2));
   alloc(ctr, 4 * sizeof(uint32_t));
   alloc(nodes, (n - 1u) * sizeof(Node8Q));
+  alloc(kids, (n - 1u) * CG_WIDE * sizeof(uint32_t));
   size_t tmp_bytes = 0;
   if (rc == CG_OK &&
       hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, static_cast<uint32_t*>(keys.p), static_cast<uint32_t*>(keys2.p),
@@ -642,6 +724,7 @@ int lbvh_build_device(cg_env* env, BlasJob& j, uint32_t node_base, const TriRec*
   A.max_children = c + 2;
   A.nodes = static_cast<Node8Q*>(nodes.p);
   A.exact = nullptr;
+  A.kids = static_cast<uint32_t*>(kids.p);
   A.node_base = node_base;
   A.slot_base = j.slot_base;
   auto fin = [&](int code) {
@@ -691,6 +774,17 @@ int lbvh_build_device(cg_env* env, BlasJob& j, uint32_t node_base, const TriRec*
   j.root = node_base;
   j.nodes = got[1];
   j.stack = stack;
+  keep.ids = ids2;
+  keep.left = left;
+  keep.right = right;
+  keep.parent = parent;
+  keep.kids = kids;
+  for (DevBuf* b : {&ids2, &left, &right, &parent, &kids}) *b = DevBuf{};
+  keep.n = n;
+  keep.slot_base = j.slot_base;
+  keep.node_base = node_base;
+  keep.nodes = j.nodes;
+  keep.stack = stack;
   return fin(CG_OK);
 }
 
@@ -933,8 +1027,9 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
     if (r != CG_OK && rc == CG_OK) rc = r;
   };
   if (on_device) {
-    // source-order triangle records, then one build per BLAS into node
-    // ranges after the host-built TLAS nodes
+    // source-order triangle records, one build per BLAS into node ranges from
+    // 0, then the host-built TLAS nodes after them (so a refit that changes the
+    // TLAS leaves the BLAS refs in place)
     DevBuf tsrc, vsrc;
     up(dev_upload(env, tsrc, L.tris.data(), L.tris.size() * sizeof(TriRec)));
     up(dev_upload(env, vsrc, L.vidx.data(), L.vidx.size() * sizeof(uint4)));
@@ -945,7 +1040,8 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
          hipMemcpyAsync(s->vidx.p, vsrc.p, vsrc.bytes, hipMemcpyDeviceToDevice, env->stream) != hipSuccess))
       rc = fail(env, CG_ERR_HIP, "scene upload failed");
     std::vector<DevBuf> built(jobs.size());
-    uint64_t node_count = L.bvhq.size();
+    s->blas.resize(jobs.size());
+    uint64_t node_count = 0;
     for (size_t k = 0; k < jobs.size() && rc == CG_OK; ++k) {
       if (node_count >= CG_REF_NONE) {
         rc = fail(env, CG_ERR_UNSUPPORTED, "too many BVH nodes");
@@ -953,14 +1049,17 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
       }
       rc = lbvh_build_device(env, jobs[k], static_cast<uint32_t>(node_count), static_cast<const TriRec*>(tsrc.p),
                              static_cast<const uint4*>(vsrc.p), static_cast<TriRec*>(s->tris.p),
-                             static_cast<uint4*>(s->vidx.p), built[k]);
+                             static_cast<uint4*>(s->vidx.p), built[k], s->blas[k]);
       node_count += jobs[k].nodes;
     }
+    const uint64_t tlas_base = node_count;
+    node_count += L.bvhq.size();
     if (rc == CG_OK && node_count >= CG_REF_NONE) rc = fail(env, CG_ERR_UNSUPPORTED, "too many BVH nodes");
+    if (rc == CG_OK) place_tlas(L, static_cast<uint32_t>(tlas_base));
     up(dev_alloc(env, s->bvh, node_count * sizeof(Node8Q)));
     if (rc == CG_OK && !L.bvhq.empty() &&
-        hipMemcpyAsync(s->bvh.p, L.bvhq.data(), L.bvhq.size() * sizeof(Node8Q), hipMemcpyHostToDevice,
-                       env->stream) != hipSuccess)
+        hipMemcpyAsync(static_cast<Node8Q*>(s->bvh.p) + tlas_base, L.bvhq.data(), L.bvhq.size() * sizeof(Node8Q),
+                       hipMemcpyHostToDevice, env->stream) != hipSuccess)
       rc = fail(env, CG_ERR_HIP, "scene upload failed");
     for (size_t k = 0; k < jobs.size() && rc == CG_OK; ++k)
       if (hipMemcpyAsync(static_cast<Node8Q*>(s->bvh.p) + jobs[k].root, built[k].p, jobs[k].nodes * sizeof(Node8Q),
@@ -972,6 +1071,9 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
     dev_free(vsrc);
     total += node_count * sizeof(Node8Q) + L.tris.size() * sizeof(TriRec) + L.vidx.size() * sizeof(uint4);
     if (rc == CG_OK) lbvh_finish(L, jobs);
+    s->device_built = true;
+    s->blas_nodes = static_cast<uint32_t>(tlas_base);
+    s->bvh_capacity = node_count;
   } else {
     up(upload_vec(env, s->bvh, L.bvhq, total));
     up(upload_vec(env, s->tris, L.tris, total));
@@ -979,21 +1081,7 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
   }
   if (rc == CG_OK && L.stack_depth > 4096u)
     rc = fail(env, CG_ERR_UNSUPPORTED, "BVH needs a %u-entry traversal stack (max 4096)", L.stack_depth);
-  up(upload_vec(env, s->nodes, L.nodes, total));
-  up(upload_vec(env, s->insts, L.insts, total));
-  up(upload_vec(env, s->vattr, L.vattr, total));
-  up(upload_vec(env, s->mats, L.mats, total));
-  up(upload_vec(env, s->emit, L.emit, total));
-  up(upload_vec(env, s->alias, L.alias, total));
-  up(upload_vec(env, s->emit_order, L.emit_order, total));
-  size_t env_bytes = static_cast<size_t>(sd->env_width) * sd->env_height * 4 * sizeof(float);
-  total += env_bytes;
-  up(dev_upload(env, s->env, sd->env_rgba32f, env_bytes));
-  const size_t tex_bytes = static_cast<size_t>(sd->tex_width) * sd->tex_height * 4 * sd->tex_layers;
-  if (tex_bytes) {
-    total += tex_bytes;
-    up(dev_upload(env, s->tex, sd->tex_rgba8, tex_bytes));
-  }
+  up(upload_records(env, s, sd, L, total));
   if (rc == CG_OK && hipStreamSynchronize(env->stream) != hipSuccess)
     rc = fail(env, CG_ERR_HIP, "scene upload failed");
   if (rc != CG_OK) {
@@ -1001,32 +1089,119 @@ int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out) {
     delete s;
     return rc;
   }
-  DevScene& v = s->view;
-  v.bvh = static_cast<const Node8Q*>(s->bvh.p);
-  v.nodes = static_cast<const PairNode*>(s->nodes.p);
-  v.tris = static_cast<const TriRec*>(s->tris.p);
-  v.insts = static_cast<const InstRec*>(s->insts.p);
-  v.tri_vidx = static_cast<const uint4*>(s->vidx.p);
-  v.vattr = static_cast<const VAttr*>(s->vattr.p);
-  v.mats = static_cast<const MatRec*>(s->mats.p);
-  v.mat_count = static_cast<uint32_t>(L.mats.size());
-  v.emit = static_cast<const EmitRec*>(s->emit.p);
-  v.emit_count = static_cast<uint32_t>(L.emit.size());
-  v.alias = static_cast<const float2*>(s->alias.p);
-  v.alias_count = static_cast<uint32_t>(L.alias.size());
-  v.emit_order = static_cast<const uint32_t*>(s->emit_order.p);
-  v.emit_root = L.emit_root;
-  v.env = static_cast<const float4*>(s->env.p);
-  v.env_w = sd->env_width;
-  v.env_h = sd->env_height;
-  v.tex = static_cast<const uint32_t*>(s->tex.p);
-  v.tex_w = tex_bytes ? sd->tex_width : 0u;
-  v.tex_h = tex_bytes ? sd->tex_height : 0u;
-  v.root = L.root;
-  s->stack_depth = L.stack_depth;
+  set_view(s, sd, L);
   s->bytes = total;
   env->scenes.push_back(s);
   *out = s;
+  return CG_OK;
+}
+
+int cg_scene_refit(cg_env* env, cg_scene* s, const cg_scene_data* sd) {
+  if (!env || !s || !sd) return fail(env, CG_ERR_INVALID, "cg_scene_refit: null argument");
+  if (std::find(env->scenes.begin(), env->scenes.end(), s) == env->scenes.end())
+    return fail(env, CG_ERR_INVALID, "cg_scene_refit: scene does not belong to this env");
+  if (!s->device_built) return fail(env, CG_ERR_INVALID, "cg_scene_refit: the scene was not uploaded with CG_ACCEL_DEVICE");
+  Layout L;
+  std::string err;
+  std::vector<BlasJob> jobs;
+  int rc = build_common(sd, L, err);
+  if (rc == CG_OK) rc = build_sah_accel(sd, L, err, &jobs);
+  if (rc != CG_OK) return fail(env, rc, "%s", err.c_str());
+  collapse_scene_bvh(L);
+  quantize_scene_bvh(L);
+  build_emitter_bvh(L);
+  bool same = jobs.size() == s->blas.size() && L.tris.size() * sizeof(TriRec) == s->tris.bytes;
+  for (size_t k = 0; same && k < jobs.size(); ++k)
+    same = jobs[k].boxes.size() == s->blas[k].n && jobs[k].slot_base == s->blas[k].slot_base;
+  if (!same)
+    return fail(env, CG_ERR_INVALID, "cg_scene_refit: the scene data has other triangles or BLASes than the scene");
+  CG_HIP(env, hipStreamSynchronize(env->stream));
+  const uint64_t need = uint64_t{s->blas_nodes} + L.bvhq.size();
+  if (need >= CG_REF_NONE) return fail(env, CG_ERR_UNSUPPORTED, "too many BVH nodes");
+  if (need > s->bvh_capacity) {  // a larger TLAS: move the BLAS nodes to a larger buffer
+    DevBuf nb;
+    rc = dev_alloc(env, nb, need * sizeof(Node8Q));
+    if (rc != CG_OK) return rc;
+    if (s->blas_nodes &&
+        hipMemcpyAsync(nb.p, s->bvh.p, uint64_t{s->blas_nodes} * sizeof(Node8Q), hipMemcpyDeviceToDevice, env->stream) !=
+            hipSuccess) {
+      dev_free(nb);
+      return fail(env, CG_ERR_HIP, "cg_scene_refit: copy failed");
+    }
+    CG_HIP(env, hipStreamSynchronize(env->stream));
+    dev_free(s->bvh);
+    s->bvh = nb;
+    s->bvh_capacity = need;
+  }
+  auto up = [&](int r) {
+    if (r != CG_OK && rc == CG_OK) rc = r;
+  };
+  DevBuf tsrc, vsrc;
+  up(dev_upload(env, tsrc, L.tris.data(), L.tris.size() * sizeof(TriRec)));
+  up(dev_upload(env, vsrc, L.vidx.data(), L.vidx.size() * sizeof(uint4)));
+  if (rc == CG_OK && L.tris.size() &&
+      (hipMemcpyAsync(s->tris.p, tsrc.p, tsrc.bytes, hipMemcpyDeviceToDevice, env->stream) != hipSuccess ||
+       hipMemcpyAsync(s->vidx.p, vsrc.p, vsrc.bytes, hipMemcpyDeviceToDevice, env->stream) != hipSuccess))
+    rc = fail(env, CG_ERR_HIP, "cg_scene_refit: copy failed");
+  Node8Q* bvh = static_cast<Node8Q*>(s->bvh.p);
+  auto grid = [](uint32_t items) { return dim3((items + kBlock - 1u) / kBlock); };
+  for (size_t k = 0; k < jobs.size() && rc == CG_OK; ++k) {
+    BlasState& b = s->blas[k];
+    BlasJob& j = jobs[k];
+    DevBuf prims, box, flag;
+    up(dev_upload(env, prims, j.boxes.data(), j.boxes.size() * sizeof(PrimBox)));
+    up(dev_alloc(env, box, (b.n - 1u) * sizeof(PrimBox)));
+    up(dev_alloc(env, flag, (b.n - 1u) * sizeof(uint32_t)));
+    if (rc == CG_OK) {
+      LbvhArrays A{};
+      A.n = b.n;
+      A.prims = static_cast<const PrimBox*>(prims.p);
+      A.pad = j.pad;
+      A.ids = static_cast<uint32_t*>(b.ids.p);
+      A.left = static_cast<uint32_t*>(b.left.p);
+      A.right = static_cast<uint32_t*>(b.right.p);
+      A.parent = static_cast<uint32_t*>(b.parent.p);
+      A.box = static_cast<PrimBox*>(box.p);
+      A.flag = static_cast<uint32_t*>(flag.p);
+      A.kids = static_cast<uint32_t*>(b.kids.p);
+      A.nodes = bvh + b.node_base;
+      A.node_base = b.node_base;
+      A.slot_base = b.slot_base;
+      hipLaunchKernelGGL(k_fill_u32, grid(b.n - 1u), dim3(kBlock), 0, env->stream, A.flag, 0u, b.n - 1u);
+      hipLaunchKernelGGL(k_lbvh_bounds, grid(b.n), dim3(kBlock), 0, env->stream, A);
+      hipLaunchKernelGGL(k_lbvh_refit, grid(b.nodes), dim3(kBlock), 0, env->stream, A, b.nodes);
+      hipLaunchKernelGGL(k_lbvh_gather, grid(b.n), dim3(kBlock), 0, env->stream,
+                         static_cast<const TriRec*>(tsrc.p) + b.slot_base, static_cast<const uint4*>(vsrc.p) + b.slot_base,
+                         static_cast<TriRec*>(s->tris.p) + b.slot_base, static_cast<uint4*>(s->vidx.p) + b.slot_base,
+                         A.ids, b.n);
+      if (hipStreamSynchronize(env->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+        rc = fail(env, CG_ERR_HIP, "cg_scene_refit: kernels failed");
+    }
+    dev_free(prims);
+    dev_free(box);
+    dev_free(flag);
+    j.root = b.node_base;
+    j.nodes = b.nodes;
+    j.stack = b.stack;
+  }
+  dev_free(tsrc);
+  dev_free(vsrc);
+  if (rc != CG_OK) return rc;
+  place_tlas(L, s->blas_nodes);
+  if (!L.bvhq.empty() && hipMemcpyAsync(bvh + s->blas_nodes, L.bvhq.data(), L.bvhq.size() * sizeof(Node8Q),
+                                        hipMemcpyHostToDevice, env->stream) != hipSuccess)
+    return fail(env, CG_ERR_HIP, "cg_scene_refit: TLAS upload failed");
+  lbvh_finish(L, jobs);
+  if (L.stack_depth > 4096u)
+    return fail(env, CG_ERR_UNSUPPORTED, "BVH needs a %u-entry traversal stack (max 4096)", L.stack_depth);
+  for (DevBuf* d : {&s->nodes, &s->insts, &s->vattr, &s->mats, &s->emit, &s->alias, &s->emit_order, &s->env, &s->tex})
+    dev_free(*d);
+  uint64_t total = s->bvh_capacity * sizeof(Node8Q) + L.tris.size() * sizeof(TriRec) + L.vidx.size() * sizeof(uint4);
+  rc = upload_records(env, s, sd, L, total);
+  if (rc == CG_OK && hipStreamSynchronize(env->stream) != hipSuccess) rc = fail(env, CG_ERR_HIP, "cg_scene_refit failed");
+  if (rc != CG_OK) return rc;
+  set_view(s, sd, L);
+  s->bytes = total;
   return CG_OK;
 }
 

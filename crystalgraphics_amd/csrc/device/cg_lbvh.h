@@ -50,6 +50,7 @@ struct LbvhArrays {
   uint32_t* max_children;     // most children of a node built in this pass
   Node8Q* nodes;              // wide nodes, local indices
   Node8* exact;               // nullable: the unquantised nodes too (host checks)
+  uint32_t* kids;             // nullable: per wide node its 8 binary children (kLbvhNone: empty), for refits
   uint32_t node_base;         // global index of local wide node 0
   uint32_t slot_base;         // global triangle slot of sorted triangle 0
 };
@@ -224,11 +225,31 @@ CG_QHD void lbvh_collapse(const LbvhArrays& A, uint2 entry) {
   }
   A.nodes[entry.y] = quantize_node(nd);
   if (A.exact) A.exact[entry.y] = nd;
+  if (A.kids)
+    for (int k = 0; k < CG_WIDE; ++k) A.kids[entry.y * CG_WIDE + k] = k < n ? ch[k] : kLbvhNone;
 #if defined(__HIP_DEVICE_COMPILE__)
   atomicMax(A.max_children, static_cast<uint32_t>(n));
 #else
   if (static_cast<uint32_t>(n) > *A.max_children) *A.max_children = static_cast<uint32_t>(n);
 #endif
+}
+
+// Refit (per wide node w): the same children and refs, boxes from the
+// current binary boxes (after lbvh_bounds over new triangle boxes).  Reads
+// the node's refs before writing it, so it may run in place.
+CG_QHD void lbvh_refit_wide(const LbvhArrays& A, uint32_t w) {
+  Node8 nd;
+  for (int k = 0; k < CG_WIDE; ++k) {
+    const uint32_t c = A.kids[w * CG_WIDE + k];
+    nd.ref[k] = c == kLbvhNone ? CG_REF_NONE : A.nodes[w].ref[k];
+    PrimBox b;
+    if (c != kLbvhNone) b = lbvh_child_box(A, c);
+    for (int a = 0; a < 3; ++a) {
+      nd.lo[a][k] = c != kLbvhNone ? b.lo[a] : __builtin_inff();
+      nd.hi[a][k] = c != kLbvhNone ? b.hi[a] : -__builtin_inff();
+    }
+  }
+  A.nodes[w] = quantize_node(nd);
 }
 
 }  // namespace cg

@@ -245,6 +245,7 @@ def device() -> C.CDLL:
         _sig(d, "cg_env_set_accel", I32, P, I32)
         _sig(d, "cg_stats_reset", I32, P)
         _sig(d, "cg_scene_upload", I32, P, C.POINTER(SceneDataC), C.POINTER(P))
+        _sig(d, "cg_scene_refit", I32, P, P, C.POINTER(SceneDataC))
         _sig(d, "cg_scene_release", I32, P, P)
         _sig(d, "cg_scene_footprint", I32, P, C.POINTER(C.c_uint64), C.POINTER(U32))
         _sig(d, "cg_layout_check", I32, C.POINTER(SceneDataC), I32, C.POINTER(C.c_uint64), C.POINTER(U32))

@@ -95,6 +95,14 @@ int cg_stats_reset(cg_env* env);
  * more, give CG_ERR_UNSUPPORTED. */
 int cg_scene_upload(cg_env* env, const cg_scene_data* sd, cg_scene** out);
 int cg_scene_release(cg_env* env, cg_scene* scene);
+/* Refit (no counterpart in the reference): a scene uploaded with
+ * CG_ACCEL_DEVICE takes new scene data with the same triangles and BLASes
+ * (vertex positions, instance transforms, materials, emitters may differ):
+ * every device-built BLAS keeps its tree and gets new boxes, the TLAS is
+ * rebuilt, the other records are replaced.  Waits for the env's stream;
+ * other launches on the scene must have finished.  CG_ERR_INVALID if the
+ * scene was uploaded otherwise or the structure differs. */
+int cg_scene_refit(cg_env* env, cg_scene* scene, const cg_scene_data* sd);
 /* Host-only check of the device layout cg_scene_upload would build (no GPU
  * needed): every quantised BVH node's decoded child boxes contain the exact
  * ones.  Reports the node count and the traversal stack bound. */
