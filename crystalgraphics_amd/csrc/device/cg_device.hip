@@ -545,11 +545,11 @@ __global__ __launch_bounds__(BLOCK, 4) void k_wf_shade(WfTables T, uint32_t q, S
     const uint32_t item = grab_item(wq, probe);
     if (item == kNoItem) break;
     const uint32_t i = item * 64u + (threadIdx.x & 63u);
-    bool push = false;
-    uint32_t slot = 0u;
+    bool push = false, finished = false;
+    uint32_t slot = 0u, j = 0u;
     if (i < n) {
       slot = queue[i];
-      const uint32_t j = T.slot_job[slot];
+      j = T.slot_job[slot];
       const WfJob& J = T.jobs[j];
       const DevScene& S = T.scenes[J.scene];
       uint32_t px, py;
@@ -560,7 +560,16 @@ __global__ __launch_bounds__(BLOCK, 4) void k_wf_shade(WfTables T, uint32_t q, S
       if (push)
         wf_store(T.path[slot], s);
       else
-        atomicSub(T.job_left + j, 1u);
+        finished = true;
+    }
+    // finished slots leave their jobs: one subtraction per job among the
+    // wave's lanes (neighbouring slots almost always share a job)
+    for (unsigned long long m = __ballot(finished); m != 0ull;) {
+      const uint32_t lead = static_cast<uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
+      const uint32_t lj = __shfl(j, static_cast<int>(lead));
+      const unsigned long long same = __ballot(finished && j == lj);
+      if ((threadIdx.x & 63u) == lead) atomicSub(T.job_left + lj, static_cast<uint32_t>(__popcll(same)));
+      m &= ~same;
     }
     wave_push(T.queue[q ^ 1u], T.counters + kWfQueueLen + 32u * (q ^ 1u), push, slot);
   }
