@@ -460,6 +460,15 @@ int cgh_view_to_buffer(const cgh_scene* s, const cg_camera* cam, float* xyz, siz
 // xyz[k] receives W*H*3 floats of request k.
 int cgh_submit_jobs(const cgh_scene* const* scenes, uint32_t nscenes, const uint32_t* scene_of, const cg_camera* cams,
                     const cg_conf* confs, uint32_t n, float* const* xyz, double* seconds) {
+  if ((nscenes && !scenes) || (n && (!scene_of || !cams || !confs || !xyz))) {
+    t_err = "cgh_submit_jobs: null argument";
+    return CG_ERR_INVALID;
+  }
+  for (uint32_t k = 0; k < n; ++k)
+    if (!xyz[k]) {
+      t_err = "cgh_submit_jobs: null output for request " + std::to_string(k);
+      return CG_ERR_INVALID;
+    }
   std::vector<const Scene*> sc(nscenes);
   for (uint32_t i = 0; i < nscenes; ++i) sc[i] = reinterpret_cast<const Scene*>(scenes[i]);
   std::vector<RenderRequest> reqs(n);

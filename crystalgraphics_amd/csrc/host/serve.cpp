@@ -551,6 +551,8 @@ const char* cgh_serve_last_error(void) { return t_serve_err.c_str(); }
 int cgh_server_create(const cgh_scene* const* scenes, uint32_t n, uint32_t engine, uint64_t budget_bytes,
                       double output_fraction, uint32_t max_trace_depth, uint32_t in_flight, uint32_t rounds_per_launch,
                       uint32_t block_size, uint32_t short_stack, cgh_server** out) {
+  if (!out || (n && !scenes)) return serve_fail("cgh_server_create: null argument");
+  *out = nullptr;
   std::vector<const Scene*> sc;
   for (uint32_t i = 0; i < n; ++i) sc.push_back(reinterpret_cast<const Scene*>(scenes[i]));
   ServeConf c;
@@ -583,6 +585,7 @@ static RenderRequest make_request(uint32_t scene, const cg_camera* cam, const cg
 }
 
 int cgh_server_submit(cgh_server* s, uint32_t scene, const cg_camera* cam, const cg_conf* conf, uint64_t* id) {
+  if (!s || !cam || !conf || !id) return serve_fail("cgh_server_submit: null argument");
   auto r = reinterpret_cast<Server*>(s)->Submit(make_request(scene, cam, conf));
   if (!r) return serve_fail(r.error());
   *id = *r;
@@ -605,6 +608,7 @@ static void copy_result(const RenderResult& r, cgh_result_info* info, cg_history
 // The next finished request (info->id == 0: none within the timeout).  `out`
 // (nullable) receives W*H history records; `cap` is its length.
 int cgh_server_next(cgh_server* s, double timeout_s, cgh_result_info* info, cg_history_pixel* out, size_t cap) {
+  if (!s || !info) return serve_fail("cgh_server_next: null argument");
   auto r = reinterpret_cast<Server*>(s)->Next(timeout_s);
   if (!r) return serve_fail(r.error());
   if (out && r->seeds.size() > cap) return serve_fail("cgh_server_next: output buffer too small");
@@ -614,6 +618,7 @@ int cgh_server_next(cgh_server* s, double timeout_s, cgh_result_info* info, cg_h
 
 int cgh_server_render_alone(cgh_server* s, uint32_t scene, const cg_camera* cam, const cg_conf* conf,
                             cgh_result_info* info, cg_history_pixel* out, size_t cap) {
+  if (!s || !cam || !conf || !info) return serve_fail("cgh_server_render_alone: null argument");
   auto r = reinterpret_cast<Server*>(s)->RenderAlone(make_request(scene, cam, conf));
   if (!r) return serve_fail(r.error());
   if (out && r->seeds.size() > cap) return serve_fail("cgh_server_render_alone: output buffer too small");

@@ -74,6 +74,11 @@ def test_null_arguments_are_rejected(cg):
     assert d.cg_view(None, None, None, 0) == 1
     assert d.cg_trace_closest(None, None, None, 4, None, None) == 1
     assert d.cg_stream_copy(None, None, None, 16) == 1
+    assert d.cg_scene_refit(None, None, None) == 1
+    h = native.host()
+    assert h.cgh_submit_jobs(None, 1, None, None, None, 1, None, None) == 1
+    assert h.cgh_server_submit(None, 0, None, None, None) != 0
+    assert h.cgh_server_next(None, 0.0, None, None, 0) != 0
 
 
 def test_api_error_paths(cg):
