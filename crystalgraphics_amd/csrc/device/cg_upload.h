@@ -32,6 +32,25 @@ inline int set_error(std::string& err, int code, const char* fmt, ...) {
   return code;
 }
 
+// fn(begin, end) over [0, n) in up to 16 chunks on threads (one call below
+// 2^16 items).  For loops whose iterations are independent.
+template <typename F>
+inline void par_ranges(size_t n, F&& fn) {
+  const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < (size_t{1} << 16) || hw == 1) {
+    fn(size_t{0}, n);
+    return;
+  }
+  const size_t chunk = (n + hw - 1) / hw;
+  std::vector<std::thread> pool;
+  for (size_t t = 0; t < hw; ++t) {
+    const size_t b = t * chunk, e = std::min(n, b + chunk);
+    if (b >= e) break;
+    pool.emplace_back([&fn, b, e] { fn(b, e); });
+  }
+  for (auto& th : pool) th.join();
+}
+
 // ---------------------------------------------------------------------------
 // Wavelength table: the reference's 16-step bisection over the CMF-sum CDF
 // (physics/wavelength.wesl:20-45, physics/cie.wesl:1-140) evaluated on the
@@ -197,21 +216,32 @@ inline int build_common(const cg_scene_data* sd, Layout& L, std::string& err) {
 
   // Vertices: attributes read for the final hit only.
   L.vattr.resize(sd->vertex_count);
-  for (uint32_t i = 0; i < sd->vertex_count; ++i) {
-    const cg_vertex& v = sd->vertices[i];
-    VAttr& a = L.vattr[i];
-    for (int k = 0; k < 3; ++k) a.nrm[k] = v.nrm[k];
-    a.uv0[0] = v.uv0[0];
-    a.uv0[1] = v.uv0[1];
-    a.uv1[0] = v.uv1[0];
-    a.uv1[1] = v.uv1[1];
-    a._pad = 0.0f;
-  }
-  for (uint32_t s = 0; s < sd->index_count; ++s) {
-    const cg_index& ix = sd->indices[s];
-    if (ix.i0 >= sd->vertex_count || ix.i1 >= sd->vertex_count || ix.i2 >= sd->vertex_count)
-      return set_error(err, CG_ERR_INVALID, "triangle slot %u indexes past the vertex buffer", s);
-  }
+  par_ranges(sd->vertex_count, [&](size_t b, size_t e) {
+    for (size_t i = b; i < e; ++i) {
+      const cg_vertex& v = sd->vertices[i];
+      VAttr& a = L.vattr[i];
+      for (int k = 0; k < 3; ++k) a.nrm[k] = v.nrm[k];
+      a.uv0[0] = v.uv0[0];
+      a.uv0[1] = v.uv0[1];
+      a.uv1[0] = v.uv1[0];
+      a.uv1[1] = v.uv1[1];
+      a._pad = 0.0f;
+    }
+  });
+  std::atomic<uint32_t> bad{~0u};  // lowest slot with an index past the vertex buffer
+  par_ranges(sd->index_count, [&](size_t b, size_t e) {
+    for (size_t s = b; s < e; ++s) {
+      const cg_index& ix = sd->indices[s];
+      if (ix.i0 >= sd->vertex_count || ix.i1 >= sd->vertex_count || ix.i2 >= sd->vertex_count) {
+        uint32_t cur = bad.load();
+        while (s < cur && !bad.compare_exchange_weak(cur, static_cast<uint32_t>(s))) {
+        }
+        return;
+      }
+    }
+  });
+  if (bad.load() != ~0u)
+    return set_error(err, CG_ERR_INVALID, "triangle slot %u indexes past the vertex buffer", bad.load());
   // Emitters and alias table.
   for (uint32_t i = 0; i < sd->emissive_count; ++i) {
     const cg_emissive_prim& e = sd->emissive[i];
@@ -709,11 +739,13 @@ inline int build_sah_accel(const cg_scene_data* sd, Layout& L, std::string& err,
     j.slot_base = base;
     j.pad = pad;
     j.boxes.resize(prims.size());
-    for (size_t k = 0; k < prims.size(); ++k)
-      for (int a = 0; a < 3; ++a) {
-        j.boxes[k].lo[a] = prims[k].lo[a];
-        j.boxes[k].hi[a] = prims[k].hi[a];
-      }
+    par_ranges(prims.size(), [&](size_t b, size_t e) {
+      for (size_t k = b; k < e; ++k)
+        for (int a = 0; a < 3; ++a) {
+          j.boxes[k].lo[a] = prims[k].lo[a];
+          j.boxes[k].hi[a] = prims[k].hi[a];
+        }
+    });
     jobs->push_back(std::move(j));
     return CG_REF_INST | static_cast<uint32_t>(jobs->size() - 1);
   };
@@ -731,9 +763,16 @@ inline int build_sah_accel(const cg_scene_data* sd, Layout& L, std::string& err,
   // Box padding: a small multiple of the coordinates' magnitude keeps every
   // box test conservative against the rounding of the triangle test.
   auto pad_of = [](const std::vector<SahTree::Prim>& prims) {
+    std::vector<float> part(16, 1.0f);  // a maximum: the same in any grouping
+    std::atomic<uint32_t> next{0};
+    par_ranges(prims.size(), [&](size_t b, size_t e) {
+      float m = 1.0f;
+      for (size_t i = b; i < e; ++i)
+        for (int k = 0; k < 3; ++k) m = std::max(m, std::max(std::fabs(prims[i].lo[k]), std::fabs(prims[i].hi[k])));
+      part[next.fetch_add(1) % 16] = m;
+    });
     float m = 1.0f;
-    for (const auto& p : prims)
-      for (int k = 0; k < 3; ++k) m = std::max(m, std::max(std::fabs(p.lo[k]), std::fabs(p.hi[k])));
+    for (float v : part) m = std::max(m, v);
     return 4e-5f * m;
   };
 
@@ -770,13 +809,25 @@ inline int build_sah_accel(const cg_scene_data* sd, Layout& L, std::string& err,
     if (built_root[ri] == CG_REF_NONE && !root_slots[ri].empty()) {
       const auto& sl = root_slots[ri];
       std::vector<SahTree::Prim> prims(sl.size());
-      for (size_t k = 0; k < sl.size(); ++k) prims[k] = tri_prim(sl[k]);
+      par_ranges(sl.size(), [&](size_t b, size_t e) {
+        for (size_t k = b; k < e; ++k) prims[k] = tri_prim(sl[k]);
+      });
+      // triangle records in the order `order` gives (identity when null)
+      auto append = [&](const std::vector<uint32_t>* order) {
+        const size_t base = L.tris.size();
+        L.tris.resize(base + sl.size());
+        L.vidx.resize(base + sl.size());
+        par_ranges(sl.size(), [&](size_t b, size_t e) {
+          for (size_t k = b; k < e; ++k) {
+            const uint32_t sidx = sl[order ? (*order)[k] : k];
+            L.tris[base + k] = make_tri(sd, sidx, CG_MISS_ID);
+            const cg_index& ix = sd->indices[sidx];
+            L.vidx[base + k] = make_uint4(ix.i0, ix.i1, ix.i2, 0u);
+          }
+        });
+      };
       if (jobs) {
-        for (uint32_t sidx : sl) {
-          L.tris.push_back(make_tri(sd, sidx, CG_MISS_ID));
-          const cg_index& ix = sd->indices[sidx];
-          L.vidx.push_back(make_uint4(ix.i0, ix.i1, ix.i2, 0u));
-        }
+        append(nullptr);
         built_root[ri] = add_job(prims, pad_of(prims));
         L.insts[i].root = built_root[ri];
         continue;
@@ -784,11 +835,7 @@ inline int build_sah_accel(const cg_scene_data* sd, Layout& L, std::string& err,
       SahTree t;
       t.build(prims, CG_LEAF_MAX);
       const uint32_t slot_base = static_cast<uint32_t>(L.tris.size());
-      for (uint32_t id : t.order) {
-        L.tris.push_back(make_tri(sd, sl[id], CG_MISS_ID));
-        const cg_index& ix = sd->indices[sl[id]];
-        L.vidx.push_back(make_uint4(ix.i0, ix.i1, ix.i2, 0u));
-      }
+      append(&t.order);
       built_root[ri] = 0;  // placeholder until pair nodes are placed
       const uint32_t r = emit_pairs(t, L.nodes, slot_base, nullptr, pad_of(prims));
       built_root[ri] = r;
