@@ -298,11 +298,24 @@ template <int kMode>
 CG_HD bool trav_step(const TravScene& S, TravRay& r, const TStack& stk, TravStats& st) {
   const float kInf = __builtin_inff();
   const uint32_t ref = r.ref;
-  if ((ref & (CG_REF_LEAF | CG_REF_INST)) == 0u) {
+  const bool at_node = (ref & (CG_REF_LEAF | CG_REF_INST)) == 0u;
+  const bool at_leaf = (ref & CG_REF_LEAF) != 0u;
+  const uint32_t slot = ref & CG_REF_SLOT_MASK;
+  const uint32_t cnt = ((ref >> CG_REF_COUNT_SHIFT) & 3u) + 1u;
+  // One load phase whatever the visit is, so a wave whose lanes sit at nodes,
+  // instances and leaves waits for memory once per step: a node's first 96
+  // bytes, an instance record (80 bytes), or the leaf's first two triangle
+  // records (48 bytes each).  Words past a record's end re-read its start.
+  const uint4* p = at_node ? reinterpret_cast<const uint4*>(S.bvh + ref)
+                   : at_leaf ? reinterpret_cast<const uint4*>(S.tris + slot)
+                             : reinterpret_cast<const uint4*>(S.insts + (ref & CG_REF_INDEX_MASK));
+  const bool wide = at_node || (at_leaf && cnt >= 2u);  // six words
+  const uint4* p3 = (wide || !at_leaf) ? p + 3 : p;
+  const uint4* p5 = wide ? p + 5 : p;
+  const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p3[0], w4 = p3[1], w5 = p5[0];
+  if (at_node) {
     // 128-B quantised node: origin + exponents, 8-bit child bounds
     // (lo x/y/z, hi x/y/z; 8 bytes each, byte i = child i), refs.
-    const uint4* q = reinterpret_cast<const uint4*>(S.bvh + ref);
-    const uint4 w0 = q[0], w1 = q[1], w2 = q[2], w3 = q[3], w4 = q[4], w5 = q[5];
     st.nodes++;
     const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
     const float sx = __uint_as_float((w0.w & 0xFFu) << 23), sy = __uint_as_float(((w0.w >> 8) & 0xFFu) << 23),
@@ -356,34 +369,46 @@ CG_HD bool trav_step(const TravScene& S, TravRay& r, const TStack& stk, TravStat
       r.ref = c[0];
       return false;
     }
-  } else if ((ref & CG_REF_LEAF) == 0u) {
+  } else if (!at_leaf) {
+    // InstRec: the inverse transform's columns (words 0-3), then root.
     const uint32_t ii = ref & CG_REF_INDEX_MASK;
-    const InstRec* in = S.insts + ii;
     st.insts++;
-    if (in->root != CG_REF_NONE) {
-      r.o = mat_point(in->inv, r.wo);
-      r.d = mat_dir(in->inv, r.wd);
+    if (w4.x != CG_REF_NONE) {
+      const float inv[16] = {__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z), __uint_as_float(w0.w),
+                             __uint_as_float(w1.x), __uint_as_float(w1.y), __uint_as_float(w1.z), __uint_as_float(w1.w),
+                             __uint_as_float(w2.x), __uint_as_float(w2.y), __uint_as_float(w2.z), __uint_as_float(w2.w),
+                             __uint_as_float(w3.x), __uint_as_float(w3.y), __uint_as_float(w3.z), __uint_as_float(w3.w)};
+      r.o = mat_point(inv, r.wo);
+      r.d = mat_dir(inv, r.wd);
       r.inv = dir_inv(r.d);
       r.cur_inst = ii;
       stk.push(r.sp, CG_REF_MARKER, 0.0f);
-      r.ref = in->root;
+      r.ref = w4.x;
       return false;
     }
   } else {
-    const uint32_t slot = ref & CG_REF_SLOT_MASK;
-    const uint32_t cnt = ((ref >> CG_REF_COUNT_SHIFT) & 3u) + 1u;
-    for (uint32_t i = 0; i < cnt; ++i) {
-      const TriRec* tr = S.tris + slot + i;
+    // TriRec: v0 (3 words), e1 (3), e2 (3), tri_id, inst, pad; two per load.
+    const uint4 a[2][3] = {{w0, w1, w2}, {w3, w4, w5}};
+#pragma unroll
+    for (uint32_t i = 0; i < 2u; ++i) {
+      if (i >= cnt) break;
+      const uint4 x = a[i][0], y = a[i][1], z = a[i][2];
       st.tris++;
-      TriT h = tri_test(r.o, r.d, ld3(tr->v0), ld3(tr->e1), ld3(tr->e2));
+      TriT h = tri_test(r.o, r.d, f3{__uint_as_float(x.x), __uint_as_float(x.y), __uint_as_float(x.z)},
+                        f3{__uint_as_float(x.w), __uint_as_float(y.x), __uint_as_float(y.y)},
+                        f3{__uint_as_float(y.z), __uint_as_float(y.w), __uint_as_float(z.x)});
       if (h.t > 0.0f && h.t < r.best.t) {
         r.best.t = h.t;
         r.best.u = h.u;
         r.best.v = h.v;
-        r.best.inst = tr->inst != CG_MISS_ID ? tr->inst : r.cur_inst;
+        r.best.inst = z.z != CG_MISS_ID ? z.z : r.cur_inst;
         r.best.slot = slot + i;
         if (kMode == 1 || (kMode == 2 && r.any)) return true;
       }
+    }
+    if (cnt > 2u) {  // the leaf's other triangles: the next step
+      r.ref = CG_MAKE_LEAF(slot + 2u, cnt - 2u);
+      return false;
     }
   }
   // pop
