@@ -58,6 +58,7 @@ struct cg_vbatch {
   DevBuf jobs, first, queue, ovf;
   std::vector<ViewArgs> host_jobs;  // sources of the in-flight copies: kept until reuse
   std::vector<uint32_t> host_first;
+  hipGraphExec_t exec = nullptr;  // the last launch's graph (counter reset + kernel)
   bool in_flight = false;
 };
 
@@ -935,6 +936,7 @@ int cg_env_destroy(cg_env* env) {
   for (cg_vbatch* v : env->vbatch_all) {
     if (v->stream) (void)hipStreamSynchronize(v->stream);
     for (DevBuf* d : {&v->jobs, &v->first, &v->queue, &v->ovf}) dev_free(*d);
+    if (v->exec) (void)hipGraphExecDestroy(v->exec);
     if (v->done) (void)hipEventDestroy(v->done);
     if (v->stream) (void)hipStreamDestroy(v->stream);
     delete v;
@@ -1460,6 +1462,16 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
       hipMemcpyAsync(v->first.p, v->host_first.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
                      v->stream) != hipSuccess)
     return give_back(fail(env, CG_ERR_HIP, "cg_view_batch_launch: job table copy failed"));
+  // The counter reset (a kernel, not a memset) and the persistent kernel are
+  // captured as one hipGraph and launched as such.  The vbatch came from the
+  // pool, so its previous graph has finished.
+  if (v->exec) {
+    (void)hipGraphExecDestroy(v->exec);
+    v->exec = nullptr;
+  }
+  (void)hipGetLastError();
+  if (hipStreamBeginCapture(v->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return give_back(fail(env, CG_ERR_HIP, "cg_view_batch_launch: graph capture failed to begin"));
   uint32_t* ctr = static_cast<uint32_t*>(v->queue.p);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kXcds * kCounterStride), 0, v->stream, ctr, kXcds * kCounterStride);
   const WorkQueue q{ctr, items, CG_VB_LPT ? 1u : 0u};
@@ -1472,7 +1484,17 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
     launch_view_batch_mf<16>(v, grid, n, q, sa, stats);
   else
     launch_view_batch_mf<64>(v, grid, n, q, sa, stats);
-  if (hipGetLastError() != hipSuccess || hipEventRecord(v->done, v->stream) != hipSuccess)
+  const hipError_t le = hipGetLastError();
+  hipGraph_t graph = nullptr;
+  const hipError_t ce = hipStreamEndCapture(v->stream, &graph);
+  const bool made = le == hipSuccess && ce == hipSuccess && graph &&
+                    hipGraphInstantiate(&v->exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (graph) (void)hipGraphDestroy(graph);
+  if (!made) {
+    v->exec = nullptr;
+    return give_back(fail(env, CG_ERR_HIP, "cg_view_batch_launch: graph capture failed"));
+  }
+  if (hipGraphLaunch(v->exec, v->stream) != hipSuccess || hipEventRecord(v->done, v->stream) != hipSuccess)
     return give_back(fail(env, CG_ERR_HIP, "cg_view_batch_launch: launch failed"));
   v->in_flight = true;
   *out = v;

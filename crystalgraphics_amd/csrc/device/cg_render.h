@@ -64,7 +64,8 @@ CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, const TStac
     for (;;) {
       HitRec h = traverse<false>(S, ro, rd, kMax, stk, st);
       ++nrays;
-      hash = hmix(hmix(hash, h.inst), h.inst != CG_MISS_ID ? S.tris[h.slot].tri_id : CG_MISS_ID);
+      if (A.path_hash)  // the triangle id is read for the path record only
+        hash = hmix(hmix(hash, h.inst), h.inst != CG_MISS_ID ? S.tris[h.slot].tri_id : CG_MISS_ID);
       bool descended = false;
       if (h.inst == CG_MISS_ID) {
         uint32_t texel = 0u;

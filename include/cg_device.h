@@ -186,7 +186,8 @@ int cg_view_job(cg_env* env, const cg_job* job);
 /* Many jobs in one persistent launch of the plain per-pixel kernel: the work
  * items are the 8x8 tiles of all n jobs (the most samples x trace depth per
  * pixel first), so the tiles of other jobs fill the GPU while one job's heavy
- * tiles finish.  Each job's
+ * tiles finish.  The counter reset and the kernel are captured as one
+ * hipGraph and launched as such.  Each job's
  * result equals cg_view_job's, bit for bit.  The launch runs on a stream of
  * its own (taken from a pool; default priority), so successive launches
  * overlap; query / wait on the returned handle, then release it.  The job
