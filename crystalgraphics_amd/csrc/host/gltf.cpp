@@ -16,11 +16,14 @@
 //     turns glTF's y-up into z-up (:969-1000);
 //   * textures: images decoded to RGBA8 (PNG here: zlib inflate and the five
 //     PNG filters; 16-bit channels keep their high byte), samplers' wrap
-//     modes kept for the emissive buffer.
+//     modes kept for the emissive buffer;
+//   * environment: an EXT_lights_image_based cube turned equirectangular, or
+//     an HDR file named in extras/extensions (Radiance RGBE or PNG, decoded as
+//     stb_image's stbi_loadf does), else the fallback sky.
 // Not supported, with an error rather than a different picture: JPEG, KTX2 and
-// WebP images, interlaced PNGs, sparse accessors, non-triangle primitives,
-// and the environment sources (an HDR file named in extras/extensions, or an
-// image-based-lighting cube) whose decoders the reference takes from stb.
+// WebP images (material textures and cube faces), interlaced PNGs, sparse
+// accessors, non-triangle primitives.  An environment HDR file in another
+// format leaves the fallback sky, as an undecodable one does in the reference.
 #include <zlib.h>
 
 #include <algorithm>
@@ -237,6 +240,211 @@ bool decode_png(const std::vector<uint8_t>& f, HostTexture& t, std::string& err)
 }
 
 // ---------------------------------------------------------------------------
+// Four-channel float images, as stb_image's stbi_loadf(..., 4) returns them
+// (the reference's DecodeImageRgba32f, src/image_decode.cpp:51-116):
+//   * Radiance RGBE (.hdr): a "#?RADIANCE" or "#?RGBE" line, header lines up
+//     to an empty one (one must read FORMAT=32-bit_rle_rgbe), "-Y <h> +X <w>",
+//     then rows top first, flat RGBE quadruples when w < 8 or w >= 32768, else
+//     new-style run-length rows (2, 2, w as two bytes, then each of the four
+//     components as runs: a count above 128 repeats the next byte count - 128
+//     times, otherwise count literal bytes follow).  A row that does not
+//     start with that marker makes those four bytes pixel 0 of row 0 and the
+//     rest of the data flat pixels from pixel 1 of row 0 on (stb's reading).
+//     A pixel is c * 2^(e - 136) per colour channel (zero when e = 0), alpha 1;
+//   * PNG: the RGBA8 image, colour channels as powf(v / 255, 2.2), alpha v / 255.
+// Truncated data is an error here (stb would return unset pixels).
+struct FloatImage {
+  uint32_t w = 0, h = 0;
+  std::vector<float> rgba;
+};
+
+void rgbe_pixel(const uint8_t* q, float* o) {
+  if (q[3] != 0) {
+    const float f1 = std::ldexp(1.0f, static_cast<int>(q[3]) - 136);
+    o[0] = static_cast<float>(q[0]) * f1;
+    o[1] = static_cast<float>(q[1]) * f1;
+    o[2] = static_cast<float>(q[2]) * f1;
+  } else {
+    o[0] = o[1] = o[2] = 0.0f;
+  }
+  o[3] = 1.0f;
+}
+
+bool is_hdr(const std::vector<uint8_t>& f) {
+  auto starts = [&](const char* sig) {
+    const size_t n = std::strlen(sig);
+    return f.size() >= n && std::memcmp(f.data(), sig, n) == 0;
+  };
+  return starts("#?RADIANCE\n") || starts("#?RGBE\n");
+}
+
+bool decode_hdr(const std::vector<uint8_t>& f, FloatImage& img, std::string& err) {
+  size_t p = 0;
+  // A header line without its newline; long lines keep their first 1023 bytes.
+  auto line = [&]() {
+    std::string t;
+    while (p < f.size() && f[p] != '\n') {
+      if (t.size() < 1023) t.push_back(static_cast<char>(f[p]));
+      ++p;
+    }
+    if (p < f.size()) ++p;
+    return t;
+  };
+  const std::string magic = line();
+  if (magic != "#?RADIANCE" && magic != "#?RGBE") return err = "not a Radiance HDR image", false;
+  bool format = false;
+  for (;;) {
+    if (p >= f.size()) return err = "truncated HDR header", false;
+    const std::string t = line();
+    if (t.empty()) break;
+    if (t == "FORMAT=32-bit_rle_rgbe") format = true;
+  }
+  if (!format) return err = "unsupported HDR format", false;
+  const std::string res = line();
+  if (res.compare(0, 3, "-Y ") != 0) return err = "unsupported HDR data layout", false;
+  const char* c = res.c_str() + 3;
+  char* e = nullptr;
+  const long h = std::strtol(c, &e, 10);
+  c = e;
+  while (*c == ' ') ++c;
+  if (std::strncmp(c, "+X ", 3) != 0) return err = "unsupported HDR data layout", false;
+  const long w = std::strtol(c + 3, nullptr, 10);
+  if (w <= 0 || h <= 0 || w > (1L << 24) || h > (1L << 24) || w * h > (1L << 28))
+    return err = "bad HDR image size", false;
+  img.w = static_cast<uint32_t>(w);
+  img.h = static_cast<uint32_t>(h);
+  img.rgba.assign(static_cast<size_t>(w) * static_cast<size_t>(h) * 4u, 0.0f);
+  const size_t n = static_cast<size_t>(w) * static_cast<size_t>(h);
+  auto flat_from = [&](size_t first) {
+    if ((n - first) * 4u > f.size() - p) return err = "truncated HDR data", false;
+    for (size_t i = first; i < n; ++i, p += 4) rgbe_pixel(&f[p], &img.rgba[i * 4u]);
+    return true;
+  };
+  if (w < 8 || w >= 32768) return flat_from(0);
+  std::vector<uint8_t> row(static_cast<size_t>(w) * 4u);
+  for (long y = 0; y < h; ++y) {
+    if (f.size() - p < 4) return err = "truncated HDR data", false;
+    if (f[p] != 2 || f[p + 1] != 2 || (f[p + 2] & 0x80)) {
+      rgbe_pixel(&f[p], &img.rgba[0]);
+      p += 4;
+      return flat_from(1);
+    }
+    if (((static_cast<long>(f[p + 2]) << 8) | f[p + 3]) != w) return err = "invalid decoded HDR scanline length", false;
+    p += 4;
+    for (int k = 0; k < 4; ++k) {
+      long x = 0;
+      while (x < w) {
+        if (p >= f.size()) return err = "truncated HDR data", false;
+        int count = f[p++];
+        const bool run = count > 128;
+        if (run) count -= 128;
+        if (count == 0 || count > w - x) return err = "bad RLE data in HDR", false;
+        if (run) {
+          if (p >= f.size()) return err = "truncated HDR data", false;
+          const uint8_t v = f[p++];
+          for (int z = 0; z < count; ++z) row[static_cast<size_t>(x++) * 4u + static_cast<size_t>(k)] = v;
+        } else {
+          if (f.size() - p < static_cast<size_t>(count)) return err = "truncated HDR data", false;
+          for (int z = 0; z < count; ++z) row[static_cast<size_t>(x++) * 4u + static_cast<size_t>(k)] = f[p++];
+        }
+      }
+    }
+    for (long x = 0; x < w; ++x)
+      rgbe_pixel(&row[static_cast<size_t>(x) * 4u], &img.rgba[(static_cast<size_t>(y) * static_cast<size_t>(w) +
+                                                              static_cast<size_t>(x)) * 4u]);
+  }
+  return true;
+}
+
+bool decode_float_image(const std::vector<uint8_t>& f, FloatImage& img, std::string& err) {
+  if (is_hdr(f)) return decode_hdr(f, img, err);
+  HostTexture t;
+  if (!decode_png(f, t, err)) return false;
+  img.w = t.width;
+  img.h = t.height;
+  img.rgba.resize(t.rgba8.size());
+  for (size_t i = 0; i < t.rgba8.size(); ++i)
+    img.rgba[i] = (i & 3u) == 3u ? static_cast<float>(t.rgba8[i]) / 255.0f
+                                 : std::pow(static_cast<float>(t.rgba8[i]) / 255.0f, 2.2f) * 1.0f;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// An EXT_lights_image_based cube as an equirectangular environment texture
+// (src/scene.cpp:461-583): 4 w x 2 h texels for w x h faces; texel (x, y)
+// looks along the engine direction of polar angle pi (y + 1/2) / (2 h) from
+// +z and azimuth 2 pi ((x + 1/2) / (4 w) - 1/2), taken to glTF axes
+// (x, -z, y), turned by the inverse of the light's rotation quaternion, and
+// reads the nearest texel of the face it falls on.  Single-precision
+// arithmetic in the reference's order of operations.
+struct V3f {
+  float x, y, z;
+};
+V3f cross3(V3f a, V3f b) { return V3f{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y}; }
+V3f rotate(const float q[4], V3f d) {
+  V3f im{q[0], q[1], q[2]};
+  float re = q[3];
+  const float norm = ((im.x * im.x + im.y * im.y) + im.z * im.z) + re * re;
+  if (norm <= 0.0f) return d;
+  const float sn = std::sqrt(norm);
+  im = V3f{im.x / sn, im.y / sn, im.z / sn};
+  re = re / sn;
+  const V3f c = cross3(im, d);
+  const V3f t{2.0f * c.x, 2.0f * c.y, 2.0f * c.z};
+  const V3f u = cross3(im, t);
+  return V3f{(d.x + re * t.x) + u.x, (d.y + re * t.y) + u.y, (d.z + re * t.z) + u.z};
+}
+const float* cube_texel(const FloatImage& img, float u, float v) {
+  u = std::clamp(u, 0.0f, 1.0f);
+  v = std::clamp(v, 0.0f, 1.0f);
+  const uint32_t x = std::min(img.w - 1u, static_cast<uint32_t>(u * static_cast<float>(img.w)));
+  const uint32_t y = std::min(img.h - 1u, static_cast<uint32_t>(v * static_cast<float>(img.h)));
+  return &img.rgba[(static_cast<size_t>(y) * img.w + x) * 4u];
+}
+const float* sample_cube(const FloatImage* faces, V3f d) {
+  const float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+  int face;
+  float u, v;
+  if (ax >= ay && ax >= az) {
+    face = d.x >= 0.0f ? 1 : 0;
+    u = d.x >= 0.0f ? (-d.z / ax + 1.0f) * 0.5f : (d.z / ax + 1.0f) * 0.5f;
+    v = (-d.y / ax + 1.0f) * 0.5f;
+  } else if (ay >= az) {
+    face = d.y >= 0.0f ? 2 : 3;
+    u = (d.x / ay + 1.0f) * 0.5f;
+    v = d.y >= 0.0f ? (d.z / ay + 1.0f) * 0.5f : (-d.z / ay + 1.0f) * 0.5f;
+  } else {
+    face = d.z >= 0.0f ? 4 : 5;
+    u = d.z >= 0.0f ? (d.x / az + 1.0f) * 0.5f : (-d.x / az + 1.0f) * 0.5f;
+    v = (1.0f - d.y / az) * 0.5f;
+  }
+  return cube_texel(faces[face], 1.0f - u, 1.0f - v);
+}
+void cube_to_env(const FloatImage* faces, const float rot[4], Scene::Impl& s) {
+  const float pi = 3.14159265358979323846f;
+  const float inv[4] = {-rot[0], -rot[1], -rot[2], rot[3]};
+  s.env_w = faces[0].w * 4u;
+  s.env_h = faces[0].h * 2u;
+  s.env_rgba32f.assign(static_cast<size_t>(s.env_w) * s.env_h * 4u, 0.0f);
+  for (uint32_t y = 0; y < s.env_h; ++y) {
+    const float v = (static_cast<float>(y) + 0.5f) / static_cast<float>(s.env_h);
+    const float theta = v * pi;
+    const float sin_theta = std::sin(theta), z = std::cos(theta);
+    for (uint32_t x = 0; x < s.env_w; ++x) {
+      const float u = (static_cast<float>(x) + 0.5f) / static_cast<float>(s.env_w);
+      const float phi = ((u - 0.5f) * 2.0f) * pi;
+      const V3f eng{std::cos(phi) * sin_theta, std::sin(phi) * sin_theta, z};
+      const float* c = sample_cube(faces, rotate(inv, V3f{eng.x, -eng.z, eng.y}));
+      float* o = &s.env_rgba32f[(static_cast<size_t>(y) * s.env_w + x) * 4u];
+      o[0] = c[0];
+      o[1] = c[1];
+      o[2] = c[2];
+      o[3] = 1.0f;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 struct Doc {
   Value root;
   std::vector<std::vector<uint8_t>> buffers;
@@ -343,6 +551,20 @@ uint32_t read_index(const Accessor& a, size_t i) {
     case 5125: { uint32_t u; std::memcpy(&u, p, 4); return u; }
     default: return 0u;
   }
+}
+
+// An image's encoded bytes: its buffer view, or its URI (src/scene.cpp:170-200).
+bool image_bytes(const Doc& d, const Value& img, std::vector<uint8_t>& bytes, std::string& mime, std::string& err) {
+  mime = img.get("mimeType") ? img.get("mimeType")->str : "";
+  if (const Value* bv = img.get("bufferView")) {
+    const uint8_t* p;
+    size_t len, stride;
+    if (!view_bytes(d, as_int(bv, -1), p, len, stride, err)) return false;
+    bytes.assign(p, p + len);
+    return true;
+  }
+  if (const Value* uri = img.get("uri")) return load_uri(uri->str, d.base, bytes, mime, err);
+  return err = "image has no source", false;
 }
 
 cg_tex_info tex_info(const Value* view, const char* scale_key) {  // LoadTextureInfo
@@ -493,26 +715,48 @@ M4d affine_inverse(const M4d& a) {
   return r;
 }
 
-// An environment source the reference would load (src/scene.cpp:316-360,
-// 586-624): an HDR path under one of its keys in the extras or extensions of
-// the file or of the scene, or an EXT_lights_image_based cube.
-bool has_environment(const Value& root, const Value& scene) {
-  static const char* keys[] = {"environment_hdr", "environmentHdr", "environmentHDR", "environmentMap", "hdr", "hdrFile"};
-  std::function<bool(const Value&)> scan = [&](const Value& v) -> bool {
-    if (v.kind == Value::kObject) {
-      for (const auto& m : v.obj) {
-        for (const char* k : keys)
-          if (m.first == k && m.second.kind == Value::kString && !m.second.str.empty()) return true;
-        if (scan(m.second)) return true;
-      }
+// The first member named `key` in document order at any depth whose value is
+// a string (want_string) or an array.
+const Value* find_member(const Value& v, const char* key, bool want_string) {
+  if (v.kind == Value::kObject) {
+    for (const auto& m : v.obj) {
+      if (m.first == key && m.second.kind == (want_string ? Value::kString : Value::kArray)) return &m.second;
+      if (const Value* r = find_member(m.second, key, want_string)) return r;
     }
-    return false;
+  } else if (v.kind == Value::kArray) {
+    for (const Value& e : v.arr)
+      if (const Value* r = find_member(e, key, want_string)) return r;
+  }
+  return nullptr;
+}
+
+// The environment HDR file (src/scene.cpp:307-359): the sources in order are
+// the file's extras, each of its extensions, then the scene's extras and each
+// of its extensions; within a source the keys in the order below, each
+// taking its first string value; an empty value passes to the next key.  The
+// path is percent-decoded and taken relative to the glTF file's folder.
+std::string environment_path(const Value& root, const Value& scene, const std::string& base) {
+  static const char* keys[] = {"environment_hdr", "environmentHdr", "environmentHDR", "environmentMap", "hdr", "hdrFile"};
+  std::vector<const Value*> sources;
+  // cgltf keeps the root extensions it parses itself out of the list.
+  auto add = [&](const Value& owner, bool is_root) {
+    if (const Value* x = owner.get("extras")) sources.push_back(x);
+    if (const Value* e = owner.get("extensions"))
+      if (e->kind == Value::kObject)
+        for (const auto& m : e->obj)
+          if (!is_root || (m.first != "KHR_lights_punctual" && m.first != "KHR_materials_variants"))
+            sources.push_back(&m.second);
   };
-  const Value* ext = root.get("extensions");
-  if (ext && ext->get("EXT_lights_image_based")) return true;
-  for (const Value* v : {root.get("extras"), ext, scene.get("extras"), scene.get("extensions")})
-    if (v && scan(*v)) return true;
-  return false;
+  add(root, true);
+  add(scene, false);
+  for (const Value* src : sources)
+    for (const char* k : keys) {
+      const Value* v = find_member(*src, k, true);
+      if (!v || v->str.empty()) continue;
+      const std::string path = percent_decode(v->str);
+      return (!path.empty() && path[0] == '/') ? path : base + path;
+    }
+  return "";
 }
 
 }  // namespace
@@ -557,8 +801,6 @@ std::string load_gltf(const std::string& path, Scene::Impl& s) {
   const Value* scenes = d.root.get("scenes");
   const Value* scene = scenes ? scenes->at(static_cast<size_t>(as_int(d.root.get("scene"), 0))) : nullptr;
   if (!scene) return "glTF file has no scene";
-  if (has_environment(d.root, *scene))
-    return "environment HDR files and image-based-lighting cubes are not supported (no image decoder)";
   std::string err;
 
   // buffers
@@ -586,6 +828,55 @@ std::string load_gltf(const std::string& path, Scene::Impl& s) {
   s.env_rgba32f.clear();
   s.env_w = s.env_h = 0;
 
+  // environment (src/scene.cpp:586-624, 853-859; scene_data.h:52-69): an
+  // EXT_lights_image_based cube (its first specular image set and rotation)
+  // wins; else the HDR file named in extras or extensions, decoded now; an HDR
+  // file that cannot be read or decoded leaves the fallback sky, as the
+  // reference's scene data does.
+  const std::string hdr_path = environment_path(d.root, *scene, d.base);
+  const Value* root_ext = d.root.get("extensions");
+  if (const Value* ibl = root_ext ? root_ext->get("EXT_lights_image_based") : nullptr) {
+    const Value* sets = find_member(*ibl, "specularImages", false);
+    const Value* first = sets ? sets->at(0) : nullptr;
+    int idx[6];
+    bool ok = first && first->size() >= 6;
+    for (int i = 0; ok && i < 6; ++i) {
+      const Value* v = first->at(static_cast<size_t>(i));
+      ok = v->is_num() && v->num >= 0.0 && v->num < 2147483647.0 && v->num == std::floor(v->num) &&
+           v->str.find_first_not_of("0123456789") == std::string::npos;
+      if (ok) idx[i] = static_cast<int>(v->num);
+    }
+    if (ok) {
+      float rot[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+      const Value* r = find_member(*ibl, "rotation", false);
+      if (r && r->size() >= 4 && r->at(0)->is_num() && r->at(1)->is_num() && r->at(2)->is_num() && r->at(3)->is_num())
+        for (int i = 0; i < 4; ++i) rot[i] = std::stof(r->at(static_cast<size_t>(i))->str);
+      FloatImage faces[6];
+      for (int i = 0; i < 6; ++i) {
+        const Value* img = arr_at(d.root, "images", idx[i]);
+        if (!img) return "image based light references an invalid image";
+        std::vector<uint8_t> bytes;
+        std::string mime;
+        if (!image_bytes(d, *img, bytes, mime, err)) return "image based light: " + err;
+        if (!decode_float_image(bytes, faces[i], err)) return "failed to decode image based light face as float: " + err;
+        if (faces[i].w == 0 || faces[i].h == 0) return "image based light has an empty cube face";
+        if (i > 0 && (faces[i].w != faces[0].w || faces[i].h != faces[0].h))
+          return "image based light cube faces do not share one size";
+      }
+      cube_to_env(faces, rot, s);
+    }
+  }
+  if (s.env_rgba32f.empty() && !hdr_path.empty()) {
+    std::vector<uint8_t> bytes;
+    FloatImage env;
+    std::string ignored;
+    if (read_file(hdr_path, bytes) && decode_float_image(bytes, env, ignored) && env.w && env.h) {
+      s.env_w = env.w;
+      s.env_h = env.h;
+      s.env_rgba32f = std::move(env.rgba);
+    }
+  }
+
   // textures (src/scene.cpp:196-230)
   if (const Value* texs = d.root.get("textures")) {
     for (size_t i = 0; i < texs->size(); ++i) {
@@ -598,17 +889,8 @@ std::string load_gltf(const std::string& path, Scene::Impl& s) {
       const Value* img = arr_at(d.root, "images", as_int(t->get("source"), -1));
       if (!img) return "texture " + std::to_string(i) + " has no image source";
       std::vector<uint8_t> bytes;
-      std::string mime = img->get("mimeType") ? img->get("mimeType")->str : "";
-      if (const Value* bv = img->get("bufferView")) {
-        const uint8_t* p;
-        size_t len, stride;
-        if (!view_bytes(d, as_int(bv, -1), p, len, stride, err)) return err;
-        bytes.assign(p, p + len);
-      } else if (const Value* uri = img->get("uri")) {
-        if (!load_uri(uri->str, d.base, bytes, mime, err)) return err;
-      } else {
-        return "texture image " + std::to_string(i) + " has no source";
-      }
+      std::string mime;
+      if (!image_bytes(d, *img, bytes, mime, err)) return "texture " + std::to_string(i) + ": " + err;
       if (!decode_png(bytes, ht, err)) return "texture " + std::to_string(i) + ": " + err + (mime.empty() ? "" : " (" + mime + ")");
       s.textures.push_back(std::move(ht));
     }
@@ -737,6 +1019,7 @@ std::string load_gltf(const std::string& path, Scene::Impl& s) {
   mix(file.data(), file.size());
   for (const auto& b : d.buffers) mix(b.data(), b.size());
   for (const auto& t : s.textures) mix(t.rgba8.data(), t.rgba8.size());
+  mix(reinterpret_cast<const uint8_t*>(s.env_rgba32f.data()), s.env_rgba32f.size() * sizeof(float));
   s.tag = h ^ std::hash<std::string>{}(path);
   return "";
 }

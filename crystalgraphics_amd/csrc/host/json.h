@@ -18,7 +18,7 @@ struct Value {
   enum Kind { kNull, kBool, kNumber, kString, kArray, kObject } kind = kNull;
   bool b = false;
   double num = 0.0;
-  std::string str;
+  std::string str;  // a string's value; a number's text as written
   std::vector<Value> arr;
   std::vector<std::pair<std::string, Value>> obj;
 
@@ -133,7 +133,7 @@ class Parser {
     }
     return fail("unterminated string");
   }
-  bool number(double& out) {
+  bool number(double& out, std::string& text) {
     const size_t b = p_;
     if (p_ < s_.size() && s_[p_] == '-') ++p_;
     while (p_ < s_.size() && ((s_[p_] >= '0' && s_[p_] <= '9') || s_[p_] == '.' || s_[p_] == 'e' || s_[p_] == 'E' ||
@@ -143,6 +143,7 @@ class Parser {
     char* end = nullptr;
     out = std::strtod(tok.c_str(), &end);
     if (tok.empty() || end != tok.c_str() + tok.size()) return fail("bad number");
+    text = tok;
     return true;
   }
   bool value(Value& v, int depth) {
@@ -218,7 +219,7 @@ class Parser {
     }
     if (c == 'n') return literal("null");
     v.kind = Value::kNumber;
-    return number(v.num);
+    return number(v.num, v.str);
   }
 };
 

@@ -1,6 +1,6 @@
 """Writers of small glTF 2.0 test files (.gltf with an external buffer and
-images, .gltf with data URIs only, .glb) and of PNG images with every row
-filter, plus the scene each file must load as, restated independently of the
+images, .gltf with data URIs only, .glb), of PNG images with every row
+filter and of Radiance RGBE (.hdr) images with flat or run-length rows, plus the scene each file must load as, restated independently of the
 library's loader: vertex and index arrays rebased per primitive, the default
 material at index 0 and glTF material i at i + 1, primitive objects in
 depth-first node order with world = Rx(+90 deg) * parent * local (local =
@@ -87,6 +87,42 @@ def png(pixels: np.ndarray, color_type: int, depth: int = 8, palette: np.ndarray
     data = zlib.compress(_filter_rows(raw, bpp))
     out += _chunk(b"IDAT", data[: len(data) // 2]) + _chunk(b"IDAT", data[len(data) // 2:])  # split IDAT
     return out + _chunk(b"IEND", b"")
+
+
+# --- Radiance RGBE (.hdr) -------------------------------------------------------------------------------------------
+def _rle_channel(vals: np.ndarray) -> bytes:
+    """New-style RLE of one component of a row: runs of >= 3 equal bytes as (128 + n, v), the rest as literals."""
+    out, i, n = bytearray(), 0, len(vals)
+    while i < n:
+        j = i
+        while j < n and vals[j] == vals[i] and j - i < 127:
+            j += 1
+        if j - i >= 3:
+            out += bytes([128 + (j - i), int(vals[i])])
+            i = j
+            continue
+        k = i
+        while k < n and k - i < 128 and not (k + 2 < n and vals[k] == vals[k + 1] == vals[k + 2]):
+            k += 1
+        k = max(k, i + 1)
+        out += bytes([k - i]) + bytes(int(v) for v in vals[i:k])
+        i = k
+    return bytes(out)
+
+
+def hdr_file(rgbe: np.ndarray, rle: bool, magic: bytes = b"#?RADIANCE", extra_header: bytes = b"",
+             res_line: bytes | None = None) -> bytes:
+    h, w = rgbe.shape[:2]
+    head = magic + b"\n# written by the tests\n" + extra_header + b"FORMAT=32-bit_rle_rgbe\nEXPOSURE=1.0\n\n"
+    head += res_line if res_line is not None else f"-Y {h} +X {w}\n".encode()
+    if not rle:
+        return head + rgbe.astype(np.uint8).tobytes()
+    body = bytearray()
+    for y in range(h):
+        body += bytes([2, 2, w >> 8, w & 255])
+        for k in range(4):
+            body += _rle_channel(rgbe[y, :, k])
+    return head + bytes(body)
 
 
 def images():

@@ -93,8 +93,6 @@ def _variant(tmp_path, mutate):
 
 @pytest.mark.parametrize("mutate,msg", [
     (lambda d: d["images"].__setitem__(0, {"uri": "data:image/jpeg;base64,/9j/4AAQ"}), "not a PNG"),
-    (lambda d: d.__setitem__("extras", {"environment_hdr": "sky.hdr"}), "environment"),
-    (lambda d: d.__setitem__("extensions", {"EXT_lights_image_based": {"lights": []}}), "environment"),
     (lambda d: d["meshes"][1]["primitives"][0].__setitem__("mode", 1), "triangle"),
     (lambda d: d["meshes"][1]["primitives"][0].__setitem__("material", 7), "material"),
     (lambda d: d["nodes"][2].__setitem__("children", [9]), "node"),

@@ -82,6 +82,39 @@ def test_gltf_scene_renders_like_the_oracle(cg, oracle, tmp_path):
     assert np.array_equal(rcount[same], ref["ray_count"][same])
 
 
+def test_gltf_environment_renders_like_the_oracle(cg, oracle, tmp_path):
+    """A glTF file whose extras name a Radiance HDR sky: LoadScene decodes it
+    into the scene's environment texture, and the device render equals the
+    oracle's on the same scene data (paths >= 99 %, XYZ within 1e-4)."""
+    import json
+
+    import gltf_files as gf
+    rng = np.random.default_rng(12)
+    sky = rng.integers(0, 256, (32, 64, 4)).astype(np.uint8)
+    sky[..., 3] = rng.integers(120, 140, (32, 64))
+    (tmp_path / "sky.hdr").write_bytes(gf.hdr_file(sky, rle=True))
+    p = gf.write(tmp_path, "datauri")
+    doc = json.loads(p.read_text())
+    doc["extras"] = {"environment_hdr": "sky.hdr"}
+    p.write_text(json.dumps(doc))
+    s = cg.LoadScene(str(p))
+    sd = s.scene_data()
+    assert s.describe()["env"].shape == (32, 64, 4)
+    cam = s.describe()["camera"]
+    conf = cg.Conf.make(80, 45, 4, 5)
+    with cg.DeviceEnv(conf) as env:
+        ds = env.upload(sd)
+        env.set_path_records(True)
+        env.view(ds, cam, 0)
+        hist = env.read_history()
+        phash, _ = env.read_path_records()
+    ref = oracle.OracleScene.wrap(sd.c).render(conf, cam)
+    same = phash == ref["path_hash"]
+    err = pixel_rel_err(hist["xyz"], ref["history"]["xyz"])
+    print(f"glTF scene with HDR sky: paths identical {same.mean() * 100:.2f}%, max rel err {err[same].max():.3e}")
+    assert same.mean() >= 0.99 and err[same].max() <= 1e-4
+
+
 def test_submit_jobs_equal_single_renders(cg):
     """SubmitJobs (many requests, two scenes, one launch) returns for every
     request exactly the image of rendering it alone with the plain kernel."""
