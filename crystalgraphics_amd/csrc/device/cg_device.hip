@@ -172,7 +172,9 @@ __device__ __forceinline__ uint32_t grab_item(const WorkQueue& q, uint32_t& prob
       ++probe;
     }
   }
-  return __shfl(item, 0);
+  // Lane 0 is active at every call (it takes the item), so it is the first
+  // active lane: the item becomes a wave-uniform (scalar) value.
+  return __builtin_amdgcn_readfirstlane(item);
 }
 
 __global__ void k_zero_u32(uint32_t* p, uint32_t n) {
