@@ -167,7 +167,7 @@ def traversal_microbench(cg, ev, n_rays: int, n_tris: int, reps: int) -> dict:
     return out
 
 
-def batched_form(cg, seed: int, n_jobs: int, budget_gib: float, log) -> dict:
+def batched_form(cg, seed: int, n_jobs: int, budget_gib: float, log, tune_engine: bool = True) -> dict:
     """The batched form: n_jobs render jobs (resolution log-uniform 64x64 to
     1920x1080, spp in {1,4,16,64}, depth 2-12, camera on a radius-3 sphere
     aimed at the origin) over 4 resident scenes built with the 16-sphere
@@ -188,7 +188,9 @@ def batched_form(cg, seed: int, n_jobs: int, budget_gib: float, log) -> dict:
     depth = max(j.depth for j in jobs)
     setup = time.perf_counter() - t0
     t0 = time.perf_counter()
-    tune = autotune.tune_batch(env, scenes, jobs, depth, log=log)
+    # tune_engine=False keeps the default shape (one graph for the whole run:
+    # profilers that trace graph launches see a single instantiation)
+    tune = autotune.tune_batch(env, scenes, jobs, depth, log=log) if tune_engine else autotune.BatchTuning()
     tune_s = time.perf_counter() - t0
     need = sum(((j.width + 7) // 8) * ((j.height + 7) // 8) * 64 for j in jobs)
     tune.slot_capacity = autotune.capacity_for(int(budget_gib * (1 << 30)), depth, need)
@@ -385,6 +387,8 @@ def main() -> int:
     ap.add_argument("--serve-texture-size", type=int, default=2048,
                     help="side of the four RGBA8 material textures per serving scene (0: untextured)")
     ap.add_argument("--batch-jobs", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched form: the wavefront engine's default shape instead of the autotuner's pick")
     ap.add_argument("--batch-budget-gib", type=float, default=16.0, help="HBM for the batch engine's slot pool")
     ap.add_argument("--probe-copy-gib", type=float, default=2.0,
                     help="size of the plain-copy HBM probe run before the frames (0: skip)")
@@ -481,7 +485,7 @@ def main() -> int:
     batched = None
     if not args.no_batched:
         batched = batched_form(cg, args.seed, args.batch_jobs, args.batch_budget_gib,
-                               lambda m: print(m, file=sys.stderr, flush=True))
+                               lambda m: print(m, file=sys.stderr, flush=True), not args.no_autotune)
 
     serving = None
     if not args.no_serve:
