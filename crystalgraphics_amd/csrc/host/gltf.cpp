@@ -309,7 +309,7 @@ bool decode_hdr(const std::vector<uint8_t>& f, FloatImage& img, std::string& err
   while (*c == ' ') ++c;
   if (std::strncmp(c, "+X ", 3) != 0) return err = "unsupported HDR data layout", false;
   const long w = std::strtol(c + 3, nullptr, 10);
-  if (w <= 0 || h <= 0 || w > (1L << 24) || h > (1L << 24) || w * h > (1L << 28))
+  if (w <= 0 || h <= 0 || w > (1L << 24) || h > (1L << 24) || w * h > (1L << 27))  // at most 2 GiB of floats
     return err = "bad HDR image size", false;
   img.w = static_cast<uint32_t>(w);
   img.h = static_cast<uint32_t>(h);
@@ -863,6 +863,8 @@ std::string load_gltf(const std::string& path, Scene::Impl& s) {
         if (i > 0 && (faces[i].w != faces[0].w || faces[i].h != faces[0].h))
           return "image based light cube faces do not share one size";
       }
+      if (uint64_t{faces[0].w} * faces[0].h * 8u > (uint64_t{1} << 27))  // the 4w x 2h texture: at most 2 GiB
+        return "image based light cube faces are too large";
       cube_to_env(faces, rot, s);
     }
   }
