@@ -356,7 +356,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_view(ViewArgs A, WorkQueue q, Sta
   const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kViewShortStack);
   const uint32_t tiles_x = (A.W + 7u) / 8u;
   TravStats st{0u, 0u, 0u};
-  uint64_t nrays = 0, nshadow = 0;
+  uint32_t nrays = 0, nshadow = 0;  // per lane; summed in 64 bits at the flush
   uint32_t probe = 0;
   const uint32_t lane = threadIdx.x & 63u;
   for (;;) {
@@ -382,7 +382,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_view_batch(const ViewArgs* __rest
   __shared__ float s_t[kShortStack * kBlock];
   const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kShortStack);
   TravStats st{0u, 0u, 0u};
-  uint64_t nrays = 0, nshadow = 0;
+  uint32_t nrays = 0, nshadow = 0;  // per lane; summed in 64 bits at the flush
   uint32_t probe = 0;
   const uint32_t lane = threadIdx.x & 63u;
   for (;;) {

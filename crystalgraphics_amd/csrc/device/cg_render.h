@@ -28,38 +28,33 @@ struct ViewArgs {
 };
 
 
+// Only what the sample loop needs stays live across it: the camera ray is
+// recomputed per sample, the wavelength leaf reread for the sample's XYZ, and
+// the running mean read at the end (the pixel is this lane's alone).
 template <int MF>
 CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, const TStack& stk, TravStats& st,
-                        uint64_t& nrays, uint64_t& nshadow) {
+                        uint32_t& nrays, uint32_t& nshadow) {
   const DevScene& S = A.S;
-  f3 co, cd;
-  camera_ray(A.cam_pos, A.cam_dir, A.viewport, A.W, A.H, px, py, co, cd);
   const uint32_t hidx = px + py * A.W;
-  const uint64_t rays_before = nrays + nshadow;  // the lane's counters span several pixels
-  const float w = 1.0f / static_cast<float>(A.iter + 1u);
-  f3 lum = mk3(0.0f, 0.0f, 0.0f);
-  uint32_t rng = hidx;
-  if (A.iter > 0u) {
-    cg_history_pixel hp = A.history[hidx];
-    lum = mk3(hp.xyz[0], hp.xyz[1], hp.xyz[2]) * (1.0f - w);
-    rng = hp.rand_seed;
-  }
+  const uint32_t rays_before = nrays + nshadow;  // the lane's counters span several pixels
+  uint32_t rng = A.iter > 0u ? A.history[hidx].rand_seed : hidx;
   rng_step(rng);
   uint32_t hash = 2166136261u;
   Frame5 F[MF];
   f3 acc = mk3(0.0f, 0.0f, 0.0f);
-  const float spp_f = static_cast<float>(A.spp);
   for (uint32_t s = 0; s < A.spp; ++s) {
     // sampleWavelength: walk the bisection heap.
     float target = clampf(rng_float(rng), 0.0f, 0.999999f);
     uint32_t node = 1u;
 #pragma unroll
     for (int k = 0; k < CG_WL_LEVELS; ++k) node = 2u * node + (A.wl_cdf[node] < target ? 1u : 0u);
-    const WlLeaf L = A.wl_leaf[node - CG_WL_LEAVES];
-    const SmitsBin sb = smits_bin(L.lambda);
+    const uint32_t leaf = node - CG_WL_LEAVES;
+    const float lambda = A.wl_leaf[leaf].lambda;
+    const SmitsBin sb = smits_bin(lambda);
     // trace(): depth-first over the transport tree.
     int depth = 0;
-    f3 ro = co, rd = cd;
+    f3 ro, rd;
+    camera_ray(A.cam_pos, A.cam_dir, A.viewport, A.W, A.H, px, py, ro, rd);
     float ret = 0.0f;
     for (;;) {
       HitRec h = traverse<false>(S, ro, rd, kMax, stk, st);
@@ -76,7 +71,7 @@ CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, const TStac
       } else {
         Surface sf = surface_at(S, ro, rd, h);
         Frame5& f = F[depth];
-        forward_pass(S, rd, L.lambda, sb, sf, depth, A.P, rng, f, stk);
+        forward_pass(S, rd, lambda, sb, sf, depth, A.P, rng, f, stk);
         if (f.ecount != 0u) {  // solveEmission (trace.wesl:91-113)
           f3 lp = sf.pos + f.edir * f.edist;
           f3 so = spawn_origin(sf.pos, sf.gn, f.edir);
@@ -120,9 +115,17 @@ CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, const TStac
       if (!next_sibling) break;
     }
     // illuminate (luminance.wesl:13-16), averaged over spp (render.wesl:13-16)
+    const WlLeaf L = A.wl_leaf[leaf];
+    const float spp_f = static_cast<float>(A.spp);
     acc.x += L.x * ret / A.wl_avg / L.prob / spp_f;
     acc.y += L.y * ret / A.wl_avg / L.prob / spp_f;
     acc.z += L.z * ret / A.wl_avg / L.prob / spp_f;
+  }
+  const float w = 1.0f / static_cast<float>(A.iter + 1u);
+  f3 lum = mk3(0.0f, 0.0f, 0.0f);
+  if (A.iter > 0u) {
+    const cg_history_pixel hp = A.history[hidx];
+    lum = mk3(hp.xyz[0], hp.xyz[1], hp.xyz[2]) * (1.0f - w);
   }
   lum = lum + acc * w;
   cg_history_pixel out;
@@ -135,7 +138,7 @@ CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, const TStac
   if (A.rgba) A.rgba[hidx] = xyz_to_rgba8(lum);
   if (A.path_hash) {
     A.path_hash[hidx] = hash;
-    A.ray_count[hidx] = static_cast<uint32_t>(nrays + nshadow - rays_before);
+    A.ray_count[hidx] = nrays + nshadow - rays_before;
   }
 }
 
