@@ -333,7 +333,10 @@ def serving_form(cg, seed: int, rate: float, seconds: float, n_scenes: int, scen
     The capacity in requests per second follows from the served rate and the
     mean queries per request; when `rate` exceeds it, requests queue and
     latency grows with the run's length, so a second run at `sustainable`
-    times the capacity reports the latency of a load the GPU keeps up with."""
+    times the capacity reports the latency of a load the GPU keeps up with.
+    The first 48 requests rendered one at a time with the plain kernel give
+    the one-at-a-time rate of the same mix, which the served rate is
+    compared with."""
     from crystalgraphics_amd import workloads
     t0 = time.perf_counter()
     params = [scene_scale, 1, texture_size] if texture_size > 0 else [scene_scale]
@@ -352,6 +355,17 @@ def serving_form(cg, seed: int, rate: float, seconds: float, n_scenes: int, scen
     out.update(serve_run(srv, reqs, check))
     per_req = out["rays"] / max(1, out["requests"])
     cap = out["Mrays/s"] * 1e6 / per_req
+    # The same request mix one job at a time with the plain kernel (the first
+    # `alone` requests of the stream, back to back, read-back included).
+    alone = min(len(reqs), 48)
+    rays0, t1 = srv.stats()["rays"], time.perf_counter()
+    for _, spec in reqs[:alone]:
+        srv.render_alone(spec.scene, spec.camera(), spec.conf())
+    t_alone = time.perf_counter() - t1
+    alone_rate = (srv.stats()["rays"] - rays0) / t_alone / 1e6
+    out["one_at_a_time"] = {"requests": alone, "seconds": round(t_alone, 3), "Mrays/s": round(alone_rate, 1)}
+    out["speedup_vs_one_at_a_time"] = round(out["Mrays/s"] / alone_rate, 3)
+    log(f"serving: the first {alone} requests one at a time: {alone_rate:.1f} Mrays/s")
     out["queries_per_request"] = round(per_req / 1e6, 2)
     out["offered_Mrays_per_s"] = round(rate * per_req / 1e6, 1)
     out["capacity_req_per_s"] = round(cap, 2)

@@ -201,6 +201,11 @@ struct Server::Impl {
       return;
     }
     arena.release(r.out);
+    cg_stats st{};
+    if (cg_stats_read(env, &st) == CG_OK) {  // cumulative: the alone render's queries count too
+      std::lock_guard<std::mutex> g(mu);
+      stats.rays = st.rays + st.shadow_rays;
+    }
     r.done_alone->set_value(finish(r, h));
   }
 
