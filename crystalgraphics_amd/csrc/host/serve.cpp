@@ -195,16 +195,18 @@ struct Server::Impl {
     }
     cg_job j = job_of(r);
     std::vector<cg_history_pixel> h(px);
+    cg_stats before{};  // the env's query counters, cumulative over its kernels
+    const bool counted = cg_stats_read(env, &before) == CG_OK;
     if (cg_view_job(env, &j) != CG_OK || cg_memcpy_d2h(env, h.data(), r.out, px * sizeof(cg_history_pixel)) != CG_OK) {
       arena.release(r.out);
       r.done_alone->set_value(unexpected<Error>(Error{last()}));
       return;
     }
     arena.release(r.out);
-    cg_stats st{};
-    if (cg_stats_read(env, &st) == CG_OK) {  // cumulative: the alone render's queries count too
+    cg_stats after{};
+    if (counted && cg_stats_read(env, &after) == CG_OK) {  // the alone render's queries count too
       std::lock_guard<std::mutex> g(mu);
-      stats.rays = st.rays + st.shadow_rays;
+      stats.rays += (after.rays + after.shadow_rays) - (before.rays + before.shadow_rays);
     }
     r.done_alone->set_value(finish(r, h));
   }
