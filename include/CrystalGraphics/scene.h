@@ -1,6 +1,7 @@
 // Scene handle (include/CrystalGraphics/scene.h:12-37 in the reference):
-// move-only, pimpl.  LoadScene accepts the procedural scenes of this library
-// ("procedural:<name>?seed=<n>") and scene files written by SaveScene.
+// move-only, pimpl.  LoadScene reads glTF 2.0 files (.gltf, .glb) as the
+// reference does, and the procedural scenes of this library
+// ("procedural:<name>?seed=<n>").
 #ifndef CRYSTALGRAPHICS_SCENE_H_
 #define CRYSTALGRAPHICS_SCENE_H_
 
