@@ -297,6 +297,12 @@ class DeviceEnv:
         self._check(native.device().cg_trace_closest(self._h, scene.handle, rays.ptr, n, hits.ptr,
                                                      uv.ptr if uv else None))
 
+    def trace_closest_noalloc_device(self, scene: DeviceScene, rays: DeviceBuffer, n: int, hits: DeviceBuffer,
+                                     uv: DeviceBuffer | None = None) -> None:
+        """cg_trace_closest_noalloc: the closest-hit batch with no device allocation inside the call."""
+        self._check(native.device().cg_trace_closest_noalloc(self._h, scene.handle, rays.ptr, n, hits.ptr,
+                                                             uv.ptr if uv else None))
+
     def trace_closest(self, scene: DeviceScene, rays: np.ndarray, with_uv: bool = False):
         """Host convenience: copy rays in, closest hits (and uv) out."""
         n = rays.shape[0]

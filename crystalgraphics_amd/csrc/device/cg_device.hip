@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,10 @@
 #include "cg_wavefront.h"
 
 using namespace cg;
+
+// Fixed device scratch of cg_trace_closest_noalloc (32 MiB per device).
+constexpr size_t kScratchBytes = size_t{32} << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
@@ -77,6 +82,7 @@ struct cg_env {
   std::vector<cg_scene*> scenes;
   std::vector<cg_vbatch*> vbatch_pool;  // finished view batches, for reuse
   std::vector<cg_vbatch*> vbatch_all;
+  void* scratch = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress)
 };
 
 namespace {
@@ -852,6 +858,33 @@ int launch_view(cg_env* env, const ViewArgs& a, int frames, uint32_t tiles, uint
   if (frames <= 16) return launch_view_mf<16>(env, a, tiles, depth);
   return launch_view_mf<64>(env, a, tiles, depth);
 }
+// ---------------------------------------------------------------------------
+// Fixed scratch of cg_trace_closest_noalloc: the work-queue counters in its
+// first kScratchQueueBytes, then the overflow traversal stack.  Every device
+// the library runs on holds one copy for the life of the process; host code
+// uses only the per-device address that hipGetSymbolAddress returns.
+constexpr size_t kScratchQueueBytes = 4096;
+static_assert(kXcds * kCounterStride * sizeof(uint32_t) <= kScratchQueueBytes, "queue counters fit the scratch head");
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Resolve this device's copy of `scratch` (the device must be current).
+int scratch_resolve(cg_env* env) {
+  if (env->device < 0 || env->device >= kMaxDevices)
+    return fail(env, CG_ERR_UNSUPPORTED, "device %d: the fixed scratch table holds %d devices", env->device,
+                kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[env->device]) {
+    void* p = nullptr;
+    CG_HIP(env, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(env, CG_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch_ptr[env->device] = p;
+  }
+  env->scratch = g_scratch_ptr[env->device];
+  return CG_OK;
+}
+
 template <bool kAnyHit>
 int launch_trace(cg_env* env, const cg_scene* scene, const cg_ray* rays, const float* tmax, uint32_t n, cg_hit* hits,
                  float* uv, uint32_t* occ, unsigned long long* st) {
@@ -863,6 +896,39 @@ int launch_trace(cg_env* env, const cg_scene* scene, const cg_ray* rays, const f
   WorkQueue q = reset_queue(env, items);
   hipLaunchKernelGGL((k_trace<kAnyHit>), dim3(grid), dim3(kBlock), 0, env->stream, scene->view, rays, tmax, n, hits, uv,
                      occ, st, q, sa);
+  return CG_OK;
+}
+
+// The closest-hit launch of launch_trace with its counters and overflow
+// stack in the fixed scratch: the grid shrinks until its overflow stacks fit
+// (the kernel is persistent, so fewer blocks only take more rays each), but
+// not below one block per CU.
+int launch_trace_noalloc(cg_env* env, const cg_scene* scene, const cg_ray* rays, uint32_t n, cg_hit* hits, float* uv,
+                         unsigned long long* st) {
+  unsigned char* base = static_cast<unsigned char*>(env->scratch);
+  if (!base) return fail(env, CG_ERR_STATE, "cg_trace_closest_noalloc: scratch not resolved for device %d", env->device);
+  const uint32_t items = (n + 63u) / 64u;
+  uint32_t grid = resident_grid(k_trace<false>, env, (items + 3u) / 4u);
+  StackArgs sa{nullptr, 0};
+  if (scene->stack_depth > static_cast<uint32_t>(kShortStack)) {
+    const size_t cap = scene->stack_depth - kShortStack;
+    const size_t per_block = static_cast<size_t>(kBlock) * cap * sizeof(uint2);
+    const size_t fit = (kScratchBytes - kScratchQueueBytes) / per_block;
+    const uint32_t floor_blocks = std::min<uint32_t>(grid, static_cast<uint32_t>(std::max(env->cu_count, 1)));
+    if (fit < floor_blocks)
+      return fail(env, CG_ERR_INVALID,
+                  "cg_trace_closest_noalloc: a traversal stack of %u entries needs %zu bytes of scratch for %u "
+                  "blocks, beyond the fixed %zu",
+                  scene->stack_depth, kScratchQueueBytes + per_block * floor_blocks, floor_blocks, kScratchBytes);
+    grid = std::min<uint32_t>(grid, static_cast<uint32_t>(fit));
+    sa.ovf = reinterpret_cast<uint2*>(base + kScratchQueueBytes);
+    sa.ovf_cap = static_cast<int>(cap);
+  }
+  uint32_t* ctr = reinterpret_cast<uint32_t*>(base);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kXcds * kCounterStride), 0, env->stream, ctr, kXcds * kCounterStride);
+  WorkQueue q{ctr, items};
+  hipLaunchKernelGGL((k_trace<false>), dim3(grid), dim3(kBlock), 0, env->stream, scene->view, rays, nullptr, n, hits,
+                     uv, nullptr, st, q, sa);
   return CG_OK;
 }
 
@@ -906,6 +972,7 @@ int cg_env_create(const cg_conf* conf, int device, cg_env** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return bail(fail(env, CG_ERR_HIP, "hipGetDeviceProperties failed"));
   env->cu_count = prop.multiProcessorCount;
+  if ((rc = scratch_resolve(env)) != CG_OK) return bail(rc);
   if (hipMemsetAsync(env->history.p, 0, env->history.bytes, env->stream) != hipSuccess ||
       hipMemsetAsync(env->stats.p, 0, env->stats.bytes, env->stream) != hipSuccess)
     return bail(fail(env, CG_ERR_HIP, "hipMemsetAsync failed"));
@@ -1564,6 +1631,17 @@ int cg_trace_closest(cg_env* env, cg_scene* scene, const cg_ray* rays, uint32_t 
   if (n == 0) return CG_OK;
   unsigned long long* st = env->counting ? static_cast<unsigned long long*>(env->stats.p) : nullptr;
   int rc = launch_trace<false>(env, scene, rays, nullptr, n, hits, uv, nullptr, st);
+  if (rc != CG_OK) return rc;
+  CG_HIP(env, hipGetLastError());
+  return CG_OK;
+}
+
+int cg_trace_closest_noalloc(cg_env* env, cg_scene* scene, const cg_ray* rays, uint32_t n, cg_hit* hits, float* uv) {
+  if (!env || !scene || (!rays && n) || (!hits && n))
+    return fail(env, CG_ERR_INVALID, "cg_trace_closest_noalloc: null argument");
+  if (n == 0) return CG_OK;
+  unsigned long long* st = env->counting ? static_cast<unsigned long long*>(env->stats.p) : nullptr;
+  int rc = launch_trace_noalloc(env, scene, rays, n, hits, uv, st);
   if (rc != CG_OK) return rc;
   CG_HIP(env, hipGetLastError());
   return CG_OK;

@@ -254,6 +254,7 @@ def device() -> C.CDLL:
         _sig(d, "cg_read_path_records", I32, P, P, P)
         _sig(d, "cg_write_history", I32, P, P)
         _sig(d, "cg_trace_closest", I32, P, P, P, U32, P, P)
+        _sig(d, "cg_trace_closest_noalloc", I32, P, P, P, U32, P, P)
         _sig(d, "cg_trace_occluded", I32, P, P, P, P, U32, P)
         _sig(d, "cg_stream_copy", I32, P, P, P, C.c_size_t)
         _sig(d, "cg_device_alloc", I32, P, C.c_size_t, C.POINTER(P))

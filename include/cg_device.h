@@ -128,6 +128,16 @@ int cg_write_history(cg_env* env, const cg_history_pixel* history);
  * (shader/compute/structural/bvh.wesl:52-130) as a standalone entry point.
  * rays / hits (and uv: 2 floats per ray, nullable) are device buffers. */
 int cg_trace_closest(cg_env* env, cg_scene* scene, const cg_ray* rays, uint32_t n, cg_hit* hits, float* uv);
+/* cg_trace_closest without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * results equal cg_trace_closest's.  Its work-queue counters and overflow
+ * traversal stack live in a fixed 32 MiB scratch array per device, resolved
+ * when the env is created; the counters are cleared by a kernel at each call.
+ * Scenes whose traversal stack would need more than that scratch for one
+ * block per CU give CG_ERR_INVALID.  All calls on one GPU share its scratch:
+ * calls that overlap on one GPU (two envs or streams of one device) race. */
+int cg_trace_closest_noalloc(cg_env* env, cg_scene* scene, const cg_ray* rays, uint32_t n, cg_hit* hits, float* uv);
 /* Batch occlusion: occluded[i] = 1 iff a hit exists with t < tmax[i]
  * (trace.wesl:104-111 asks exactly this of its shadow rays). */
 int cg_trace_occluded(cg_env* env, cg_scene* scene, const cg_ray* rays, const float* tmax, uint32_t n,
