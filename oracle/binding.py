@@ -23,6 +23,20 @@ U32 = C.c_uint32
 FP = C.POINTER(C.c_float)
 
 
+# Lobe models of the sampling code (sample.wesl:45-52).
+DIFFUSE, SPEC_REFLECT, ROUGH_REFLECT, SPEC_TRANSMIT, ROUGH_TRANSMIT = 1, 2, 3, 4, 5
+
+
+class OrcMat(C.Structure):
+    """A decoded material at one wavelength (cg_oracle.h orc_mat)."""
+    _fields_ = [("diffuse", C.c_float), ("alpha", C.c_float), ("ior_re", C.c_float), ("ior_im", C.c_float),
+                ("transmission", C.c_float), ("volume", C.c_uint32)]
+
+
+def mat(diffuse=0.0, alpha=0.0, ior=1.5, ior_im=0.0, transmission=0.0, volume=False) -> OrcMat:
+    return OrcMat(diffuse, alpha, ior, ior_im, transmission, int(bool(volume)))
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
@@ -59,6 +73,20 @@ def lib() -> C.CDLL:
             ("orc_intersect_triangle", C.c_int, [P, FP, FP, FP, FP]),
             ("orc_intersect_aabb", C.c_int, [P, FP, FP]),
             ("orc_sample_lobe", F, [C.c_int, FP, FP, F, F, F, F, C.POINTER(U32), FP]),
+            ("orc_lobe_sample", F, [C.c_int, FP, FP, C.POINTER(OrcMat), C.POINTER(U32), FP]),
+            ("orc_lobe_eval", F, [C.c_int, FP, FP, C.POINTER(OrcMat), FP, FP]),
+            ("orc_ggx_d", F, [FP, FP, F]),
+            ("orc_ggx_g1", F, [F, F]),
+            ("orc_ggx_g2", F, [F, F, F]),
+            ("orc_sample_micro_normal", None, [FP, FP, F, C.POINTER(U32), FP]),
+            ("orc_tangent_space", None, [FP, FP, FP]),
+            ("orc_reflect", None, [FP, FP, FP]),
+            ("orc_refract", None, [FP, FP, F, FP]),
+            ("orc_sample_counts", None, [C.POINTER(OrcMat), C.c_int, C.c_int, C.POINTER(Conf), C.POINTER(U32)]),
+            ("orc_balance_denominator", F, [P, FP, FP, FP, C.POINTER(OrcMat), C.POINTER(U32), C.c_int, F, F, FP]),
+            ("orc_light_pdf", F, [P, FP, FP]),
+            ("orc_emission_sample", F, [P, F, FP, C.POINTER(U32), FP, FP, FP, C.POINTER(U32)]),
+            ("orc_spawn", None, [FP, FP, FP, FP]),
         ]
         for name, res, args in sig:
             f = getattr(L, name)
@@ -83,6 +111,33 @@ def emissive_texel(image: np.ndarray, wrap_s: int, wrap_t: int, u: float, v: flo
     out = (C.c_float * 3)()
     lib().orc_emissive_texel(C.byref(t), u, v, out)
     return np.array(out[:], np.float32)
+
+
+def lobe_sample(model: int, d, n, m: OrcMat, state: int) -> tuple[np.ndarray, float, int]:
+    """(direction, pdf, next rng state) of one draw of a lobe."""
+    st = U32(state)
+    out = (C.c_float * 3)()
+    pdf = lib().orc_lobe_sample(model, fvec(d), fvec(n), C.byref(m), C.byref(st), out)
+    return np.array(out[:], np.float32), float(pdf), int(st.value)
+
+
+def lobe_eval(model: int, d, n, m: OrcMat, nxt) -> tuple[float, float]:
+    """(pdf, dist) of a lobe at a given next direction."""
+    dist = C.c_float()
+    pdf = lib().orc_lobe_eval(model, fvec(d), fvec(n), C.byref(m), fvec(nxt), C.byref(dist))
+    return float(pdf), float(dist.value)
+
+
+def vec_out(fn, *args) -> np.ndarray:
+    out = (C.c_float * 3)()
+    fn(*args, out)
+    return np.array(out[:], np.float32)
+
+
+def sample_counts(m: OrcMat, can_sample: bool, depth: int, conf: Conf) -> tuple[int, ...]:
+    out = (U32 * 6)()
+    lib().orc_sample_counts(C.byref(m), int(can_sample), depth, C.byref(conf), out)
+    return tuple(int(x) for x in out)
 
 
 def fvec(values) -> C.Array:
@@ -156,6 +211,24 @@ class OracleScene:
             raise RuntimeError("oracle: scene uses features the oracle does not restate")
         return {"history": hist.reshape(h, w), "rgba": rgba, "path_hash": ph.reshape(h, w),
                 "ray_count": rc.reshape(h, w)}
+
+    def light_pdf(self, pos, next_dir) -> float:
+        return float(lib().orc_light_pdf(self._h, fvec(pos), fvec(next_dir)))
+
+    def emission_sample(self, lam: float, pos, state: int) -> dict:
+        st = U32(state)
+        d = (C.c_float * 3)()
+        dist, rad, prim = C.c_float(), C.c_float(), U32()
+        pdf = lib().orc_emission_sample(self._h, lam, fvec(pos), C.byref(st), d, C.byref(dist), C.byref(rad),
+                                        C.byref(prim))
+        return {"dir": np.array(d[:], np.float32), "pdf": float(pdf), "dist": float(dist.value),
+                "radiance": float(rad.value), "prim": int(prim.value), "state": int(st.value)}
+
+    def balance_denominator(self, d, pos, n, m: OrcMat, counts, model: int, model_pdf: float, light_pdf: float,
+                            next_dir) -> float:
+        c = (U32 * 6)(*counts)
+        return float(lib().orc_balance_denominator(self._h, fvec(d), fvec(pos), fvec(n), C.byref(m), c, model,
+                                                   model_pdf, light_pdf, fvec(next_dir)))
 
     def close(self) -> None:
         if self._h:

@@ -952,7 +952,11 @@ uint32_t scaled_count(float h, float scale, uint32_t mx) {  // :214-219
   if (h <= 1e-6f) return 0u;
   return std::min(std::max(1u, f2u(h * scale)), mx);
 }
+Counts6 sample_counts(bool can, const Mat& m, int depth, const Params& P);
 Counts6 sample_counts(const orc_scene& sc, const Mat& m, int depth, const Params& P) {  // :110-246
+  return sample_counts(can_sample(sc), m, depth, P);
+}
+Counts6 sample_counts(bool can, const Mat& m, int depth, const Params& P) {
   float hd = m.diffuse;
   float refl = normal_reflectance(m);
   float trans = m.transmission;
@@ -960,7 +964,7 @@ Counts6 sample_counts(const orc_scene& sc, const Mat& m, int depth, const Params
   float hrr = m.alpha > 0.0f ? refl : 0.0f;
   float hst = (trans > 1e-6f && m.alpha == 0.0f) ? trans : 0.0f;
   float hrt = (trans > 1e-6f && m.alpha > 0.0f) ? trans : 0.0f;
-  float hl = can_sample(sc) ? 1.0f : 0.0f;
+  float hl = can ? 1.0f : 0.0f;
   float heur = static_cast<float>(P.D - depth) / static_cast<float>(P.D);  // lod.wesl:16-18
   uint32_t t_avail = f2u(heur * static_cast<float>(std::min(P.T, 4u)));
   uint32_t e_avail = f2u(heur * static_cast<float>(std::min(P.E, 1u)));
@@ -1755,6 +1759,153 @@ float orc_sample_lobe(int model, const float* curr_dir, const float* shading_nor
   next_dir[1] = t.dir.y;
   next_dir[2] = t.dir.z;
   return t.prob;
+}
+
+
+// ---------------------------------------------------------------------------
+// Unit-level pieces of the shading half, for the closed-form and statistical
+// checks in tests/test_oracle.py.
+}  // extern "C"
+namespace {
+Mat unit_mat(const orc_mat* m) {
+  Mat x{};
+  x.diffuse = m->diffuse;
+  x.alpha = m->alpha;
+  x.ior = IOR{m->ior_re, m->ior_im};
+  x.transmission = m->transmission;
+  x.volume = m->volume != 0u;
+  return x;
+}
+Surface unit_surface(const float* pos, const float* sn) {
+  Surface s{};
+  s.pos = pos ? mk(pos) : mk(0, 0, 0);
+  s.sn = mk(sn);
+  s.gn = s.sn;
+  return s;
+}
+void put3(float* out, v3 v) {
+  out[0] = v.x;
+  out[1] = v.y;
+  out[2] = v.z;
+}
+}  // namespace
+extern "C" {
+
+float orc_lobe_sample(int model, const float* curr_dir, const float* shading_norm, const orc_mat* mat,
+                      uint32_t* rng_state, float* next_dir) {
+  const Surface s = unit_surface(nullptr, shading_norm);
+  const Mat m = unit_mat(mat);
+  const v3 d = mk(curr_dir);
+  TSample t{mk(0, 0, 0), 0.0f};
+  switch (model) {
+    case M_DIFFUSE: t = diffuse_sample(d, s, *rng_state); break;
+    case M_SPEC_R: {  // sample.wesl:274-291 with specularReflectSample
+      v3 nd = specular_dir(d, s);
+      t = TSample{nd, spec_reflect_pdf(d, nd, s)};
+      break;
+    }
+    case M_ROUGH_R: t = rough_reflect_sample(d, s, m, *rng_state); break;
+    case M_SPEC_T: {  // sample.wesl:312-329 with specularTransmitSample
+      v3 nd = m.volume ? spec_trans_dir(d, s, m) : d;
+      t = TSample{nd, spec_trans_pdf(d, nd, s, m)};
+      break;
+    }
+    case M_ROUGH_T: t = rough_trans_sample(d, s, m, *rng_state); break;
+    default: break;
+  }
+  put3(next_dir, t.dir);
+  return t.prob;
+}
+
+float orc_lobe_eval(int model, const float* curr_dir, const float* shading_norm, const orc_mat* mat,
+                    const float* next_dir, float* dist) {
+  const Surface s = unit_surface(nullptr, shading_norm);
+  const Mat m = unit_mat(mat);
+  const v3 d = mk(curr_dir), n = mk(next_dir);
+  float pdf = 0.0f, f = 0.0f;
+  switch (model) {
+    case M_DIFFUSE: pdf = diffuse_pdf(d, n, s); f = diffuse_dist(d, n, s, m); break;
+    case M_SPEC_R: pdf = spec_reflect_pdf(d, n, s); f = spec_reflect_dist(d, n, s, m); break;
+    case M_ROUGH_R: pdf = rough_reflect_pdf(d, n, s, m); f = rough_reflect_dist(d, n, s, m); break;
+    case M_SPEC_T: pdf = spec_trans_pdf(d, n, s, m); f = spec_trans_dist(d, n, s, m); break;
+    case M_ROUGH_T: pdf = rough_trans_pdf(d, n, s, m); f = rough_trans_dist(d, n, s, m); break;
+    default: break;
+  }
+  if (dist) *dist = f;
+  return pdf;
+}
+
+float orc_ggx_d(const float* micro, const float* macro, float alpha) { return ggx_d(mk(micro), mk(macro), alpha); }
+float orc_ggx_g1(float c, float alpha) { return ggx_g1(c, alpha); }
+float orc_ggx_g2(float c1, float c2, float alpha) { return ggx_g2(c1, c2, alpha); }
+void orc_sample_micro_normal(const float* curr_dir, const float* shading_norm, float alpha, uint32_t* rng_state,
+                             float* micro) {
+  put3(micro, sample_micro_normal(mk(curr_dir), unit_surface(nullptr, shading_norm), alpha, *rng_state));
+}
+void orc_tangent_space(const float* n, float* t, float* b) {
+  Frame f = tangent_space(mk(n));
+  put3(t, f.t);
+  put3(b, f.b);
+}
+void orc_reflect(const float* d, const float* n, float* out) { put3(out, reflect(mk(d), mk(n))); }
+void orc_refract(const float* d, const float* n, float eta, float* out) { put3(out, refract(mk(d), mk(n), eta)); }
+
+void orc_sample_counts(const orc_mat* mat, int can, int depth, const cg_conf* conf, uint32_t* out) {
+  Params P;
+  P.D = std::max(1, std::min(static_cast<int>(conf->trace_depth), 64));
+  P.T = conf->max_transport_sample_count;
+  P.E = conf->max_emission_sample_count;
+  P.DIF = conf->max_diffuse_sample_count;
+  P.ROUGH = conf->max_rough_sample_count;
+  Counts6 c = sample_counts(can != 0, unit_mat(mat), depth, P);
+  out[0] = c.d;
+  out[1] = c.sr;
+  out[2] = c.rr;
+  out[3] = c.st;
+  out[4] = c.rt;
+  out[5] = c.l;
+}
+
+float orc_balance_denominator(const orc_scene* sc, const float* curr_dir, const float* pos, const float* shading_norm,
+                              const orc_mat* mat, const uint32_t* counts, int model, float model_pdf,
+                              float sampled_light_pdf, const float* next_dir) {
+  Counts6 c;
+  c.d = counts[0];
+  c.sr = counts[1];
+  c.rr = counts[2];
+  c.st = counts[3];
+  c.rt = counts[4];
+  c.l = counts[5];
+  return balance_denominator(*sc, mk(curr_dir), mk(next_dir), unit_surface(pos, shading_norm), unit_mat(mat), c, model,
+                             model_pdf, sampled_light_pdf);
+}
+
+float orc_light_pdf(const orc_scene* sc, const float* pos, const float* next_dir) {
+  return light_pdf(*sc, mk(next_dir), unit_surface(pos, next_dir));
+}
+
+float orc_emission_sample(const orc_scene* sc, float lambda, const float* pos, uint32_t* rng_state, float* next_dir,
+                          float* expected_dist, float* radiance, uint32_t* prim) {
+  if (prim) {
+    uint32_t probe = *rng_state;
+    *prim = can_sample(*sc) ? sample_emissive_prim(*sc, probe) : 0xFFFFFFFFu;
+  }
+  const float up[3] = {0.0f, 0.0f, 1.0f};  // emissionSample reads only the surface position
+  ESample e = emission_sample(*sc, lambda, unit_surface(pos, up), *rng_state);
+  put3(next_dir, e.dir);
+  *expected_dist = e.expected_dist;
+  *radiance = e.radiance;
+  return e.prob;
+}
+
+void orc_spawn(const float* pos, const float* geom_norm, const float* next_dir, float* out_pos) {
+  Surface s{};
+  s.pos = mk(pos);
+  s.gn = mk(geom_norm);
+  s.sn = s.gn;
+  v3 p, d;
+  spawn(s, mk(next_dir), p, d);
+  put3(out_pos, p);
 }
 
 }  // extern "C"

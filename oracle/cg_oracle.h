@@ -106,6 +106,51 @@ float orc_sample_lobe(int model, const float* curr_dir, const float* shading_nor
                       float alpha, float ior_re, float ior_im, float transmission,
                       uint32_t* rng_state, float* next_dir);
 
+/* A decoded material at one wavelength (material.wesl:60-67). */
+typedef struct orc_mat {
+  float diffuse, alpha, ior_re, ior_im, transmission;
+  uint32_t volume;
+} orc_mat;
+/* One lobe at a surface with shading normal sn (= geometric normal here):
+ * model 1 diffuse, 2 specular reflection, 3 rough reflection, 4 specular
+ * transmission, 5 rough transmission.  orc_lobe_sample draws a direction as
+ * sampleModels does and returns its pdf; orc_lobe_eval returns the lobe's
+ * pdf at next_dir and writes its dist (f times the projected cosine). */
+float orc_lobe_sample(int model, const float* curr_dir, const float* shading_norm, const orc_mat* m,
+                      uint32_t* rng_state, float* next_dir);
+float orc_lobe_eval(int model, const float* curr_dir, const float* shading_norm, const orc_mat* m,
+                    const float* next_dir, float* dist);
+/* GGX pieces with the reference's alpha (rough_reflection.wesl:120-139). */
+float orc_ggx_d(const float* micro, const float* macro, float alpha);
+float orc_ggx_g1(float c, float alpha);
+float orc_ggx_g2(float c1, float c2, float alpha);
+/* The visible-normal sampler (rough_reflection.wesl:52-103). */
+void orc_sample_micro_normal(const float* curr_dir, const float* shading_norm, float alpha,
+                             uint32_t* rng_state, float* micro);
+void orc_tangent_space(const float* n, float* t, float* b);  /* geometric.wesl:28-33 */
+void orc_reflect(const float* d, const float* n, float* out);  /* WGSL reflect */
+void orc_refract(const float* d, const float* n, float eta, float* out);  /* WGSL refract */
+/* Lobe counts (sample.wesl:110-246) at lod depth `depth` for conf's
+ * trace depth and caps: out = {diffuse, specular reflect, rough reflect,
+ * specular transmit, rough transmit, light}. */
+void orc_sample_counts(const orc_mat* m, int can_sample, int depth, const cg_conf* conf, uint32_t* out);
+/* balancePdfDenominator (sample.wesl:452-501) at a surface point of scene s;
+ * model / model_pdf name the sampled lobe (0: none), sampled_light_pdf >= 0
+ * replaces the light pdf. */
+float orc_balance_denominator(const orc_scene* s, const float* curr_dir, const float* pos,
+                              const float* shading_norm, const orc_mat* m, const uint32_t* counts,
+                              int model, float model_pdf, float sampled_light_pdf,
+                              const float* next_dir);
+/* lightPdf (pdf/emission.wesl:120-149) from pos along next_dir. */
+float orc_light_pdf(const orc_scene* s, const float* pos, const float* next_dir);
+/* emissionSample (pdf/emission.wesl:28-96) from pos: returns the solid-angle
+ * pdf; writes the direction, the distance to the light point, the radiance
+ * at lambda and (nullable) the picked emitter. */
+float orc_emission_sample(const orc_scene* s, float lambda, const float* pos, uint32_t* rng_state,
+                          float* next_dir, float* expected_dist, float* radiance, uint32_t* prim);
+/* spawnSurfaceRay's origin (spawn_surface_ray.wesl:6-15). */
+void orc_spawn(const float* pos, const float* geom_norm, const float* next_dir, float* out_pos);
+
 #ifdef __cplusplus
 }
 #endif
