@@ -423,9 +423,9 @@ class ViewBatch:
         self.env._check(native.device().cg_view_batch_wait(self._h))
 
     def release(self) -> None:
-        if self._h:
+        if self._h and self.env._h:  # a closed env has released its view batches
             self.env._check(native.device().cg_view_batch_release(self._h))
-            self._h = None
+        self._h = None
 
 
 class Batch:
@@ -476,9 +476,9 @@ class Batch:
         return b.value, f.value
 
     def close(self) -> None:
-        if self._h:
+        if self._h and self.env._h:  # a closed env has destroyed its batches
             native.device().cg_batch_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def __enter__(self):
         return self
@@ -498,12 +498,12 @@ class Server:
 
     def __init__(self, scenes: list[HostScene], budget_bytes: int = 16 << 30, engine: int = 0,
                  output_fraction: float = 0.25, max_trace_depth: int = 8, in_flight: int = 4,
-                 rounds_per_launch: int = 16, block_size: int = 0, short_stack: int = 0):
+                 rounds_per_launch: int = 16, block_size: int = 0, short_stack: int = 0, max_waiting: int = 0):
         arr = (C.c_void_p * len(scenes))(*[s.handle for s in scenes])
         h = C.c_void_p()
         rc = native.host().cgh_server_create(arr, len(scenes), engine, int(budget_bytes), float(output_fraction),
                                              max_trace_depth, in_flight, rounds_per_launch, block_size, short_stack,
-                                             C.byref(h))
+                                             max_waiting, C.byref(h))
         if rc != 0:
             raise NativeError(rc, native.host().cgh_serve_last_error().decode())
         self._h = h.value
@@ -513,9 +513,13 @@ class Server:
         if rc != 0:
             raise NativeError(rc, native.host().cgh_serve_last_error().decode())
 
-    def submit(self, scene: int, camera: Camera, conf: Conf) -> int:
+    def submit(self, scene: int, camera: Camera, conf: Conf) -> int | None:
+        """The request's id, or None when the server refuses it (max_waiting requests already wait)."""
         rid = C.c_uint64()
-        self._check(native.host().cgh_server_submit(self._h, scene, C.byref(camera), C.byref(conf), C.byref(rid)))
+        rc = native.host().cgh_server_submit(self._h, scene, C.byref(camera), C.byref(conf), C.byref(rid))
+        if rc == 6:
+            return None
+        self._check(rc)
         return rid.value
 
     def _result(self, info) -> dict:

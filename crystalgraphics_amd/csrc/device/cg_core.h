@@ -7,8 +7,9 @@
 // indices are clamped the way WebGPU robustness clamps them.  That makes
 // traversal, intersection, RNG, wavelength sampling, alias sampling, Smits
 // lifts, Fresnel, GGX terms and lobe sampling bit-identical to a CPU
-// restatement; only acos/atan2 (the sky texel, a discrete choice) and pow
-// (the sRGB target) may differ in the last ulp.
+// restatement; only acos/atan2 (the sky texel, a discrete choice) may differ
+// in the last ulp.  The sRGB target's power is taken in double precision and
+// rounded once to binary32, as the host does it.
 #ifndef CG_CORE_H_
 #define CG_CORE_H_
 
@@ -205,12 +206,15 @@ struct TStack {
   uint2* ovf;
   uint32_t ovf_stride;
   int ovf_cap;  // overflow entries per lane (the bound says none are needed beyond)
+  uint32_t* err;  // set to 1 if an entry is dropped past ovf_cap (nullable)
   CG_HD void push(int& sp, uint32_t r, float tt) const {
     if (sp < short_n) {
       ref[sp * stride] = r;
       t[sp * stride] = tt;
     } else if (sp - short_n < ovf_cap) {
       ovf[static_cast<size_t>(sp - short_n) * ovf_stride] = make_uint2(r, __float_as_uint(tt));
+    } else if (err) {
+      *err = 1u;  // the stack bound was exceeded: the walk is incomplete
     }
     ++sp;
   }
@@ -1167,7 +1171,11 @@ CG_HD uint32_t hmix(uint32_t h, uint32_t w) { return (h ^ w) * 16777619u; }
 // physics/srgb.wesl:5-27
 CG_HD float encode_channel(float lin) {
   float v = fmx(lin, 0.0f);
-  float e = v <= 0.0031308f ? 12.92f * v : 1.055f * powf(v, 1.0f / 2.4f) - 0.055f;
+  // pow(v, 1/2.4) with the f32 exponent, as the binary32 rounding of the
+  // double-precision power: the same bits on the host and the device
+  float e = v <= 0.0031308f ? 12.92f * v
+                            : 1.055f * static_cast<float>(pow(static_cast<double>(v),
+                                                              static_cast<double>(1.0f / 2.4f))) - 0.055f;
   return clampf(e, 0.0f, 1.0f);
 }
 CG_HD uint32_t xyz_to_rgba8(f3 xyz) {

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -83,6 +84,9 @@ struct cg_env {
   std::vector<cg_vbatch*> vbatch_pool;  // finished view batches, for reuse
   std::vector<cg_vbatch*> vbatch_all;
   void* scratch = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress)
+  DevBuf errflag;           // device error flag: a traversal stack overflowed
+  int test_stack_cap = -1;  // CG_TEST_STACK_CAP: caps overflow entries (tests force an overflow)
+  std::vector<cg_batch*> batches;  // live wavefront batches, destroyed with the env
 };
 
 namespace {
@@ -223,6 +227,7 @@ constexpr int kBlock = 256;
 struct StackArgs {
   uint2* ovf;
   int ovf_cap;
+  uint32_t* err;  // the env's device error flag (stack overflow)
 };
 // The stack of this lane: `short_n` entries in the block's LDS arrays, the
 // rest in the overflow area interleaved over the whole grid.
@@ -230,7 +235,7 @@ template <int BLOCK>
 __device__ __forceinline__ TStack lane_stack(uint32_t* s_ref, float* s_t, const StackArgs& sa, int short_n) {
   const uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
   return TStack{s_ref + threadIdx.x, s_t + threadIdx.x, BLOCK, short_n, sa.ovf ? sa.ovf + g : nullptr,
-                gridDim.x * BLOCK, sa.ovf ? sa.ovf_cap : 0};
+                gridDim.x * BLOCK, sa.ovf ? sa.ovf_cap : 0, sa.err};
 }
 
 // Persistent traversal of queued rays with lane refill: a wave takes rays
@@ -827,8 +832,11 @@ WorkQueue reset_queue(cg_env* env, uint32_t items) {
 int stack_args(cg_env* env, uint32_t grid, uint32_t depth, StackArgs& sa, int short_n = kShortStack) {
   sa.ovf = nullptr;
   sa.ovf_cap = 0;
+  sa.err = static_cast<uint32_t*>(env->errflag.p);
   if (depth <= static_cast<uint32_t>(short_n)) return CG_OK;
-  const int cap = static_cast<int>(depth) - short_n;
+  int cap = static_cast<int>(depth) - short_n;
+  if (env->test_stack_cap >= 0) cap = std::min(cap, env->test_stack_cap);
+  if (cap == 0) return CG_OK;
   const size_t bytes = static_cast<size_t>(grid) * kBlock * static_cast<size_t>(cap) * sizeof(uint2);
   if (env->ovf.bytes < bytes) {
     if (env->ovf.p) {
@@ -885,6 +893,28 @@ int scratch_resolve(cg_env* env) {
   return CG_OK;
 }
 
+// Wait for everything in flight on the env: its stream and every view
+// batch's stream.
+void wait_all(cg_env* env) {
+  if (env->stream) (void)hipStreamSynchronize(env->stream);
+  for (cg_vbatch* v : env->vbatch_all)
+    if (v->stream) (void)hipStreamSynchronize(v->stream);
+}
+
+// Report (and clear) the device error flag after the work that may set it
+// has finished.
+int check_device_error(cg_env* env) {
+  uint32_t flag = 0;
+  CG_HIP(env, hipMemcpyAsync(&flag, env->errflag.p, sizeof(flag), hipMemcpyDeviceToHost, env->stream));
+  CG_HIP(env, hipStreamSynchronize(env->stream));
+  if (!flag) return CG_OK;
+  CG_HIP(env, hipMemsetAsync(env->errflag.p, 0, sizeof(flag), env->stream));
+  CG_HIP(env, hipStreamSynchronize(env->stream));
+  return fail(env, CG_ERR_STATE,
+              "traversal stack overflow: a walk needed more entries than the scene's stack bound; results since the "
+              "last check are incomplete");
+}
+
 template <bool kAnyHit>
 int launch_trace(cg_env* env, const cg_scene* scene, const cg_ray* rays, const float* tmax, uint32_t n, cg_hit* hits,
                  float* uv, uint32_t* occ, unsigned long long* st) {
@@ -909,7 +939,7 @@ int launch_trace_noalloc(cg_env* env, const cg_scene* scene, const cg_ray* rays,
   if (!base) return fail(env, CG_ERR_STATE, "cg_trace_closest_noalloc: scratch not resolved for device %d", env->device);
   const uint32_t items = (n + 63u) / 64u;
   uint32_t grid = resident_grid(k_trace<false>, env, (items + 3u) / 4u);
-  StackArgs sa{nullptr, 0};
+  StackArgs sa{nullptr, 0, static_cast<uint32_t*>(env->errflag.p)};
   if (scene->stack_depth > static_cast<uint32_t>(kShortStack)) {
     const size_t cap = scene->stack_depth - kShortStack;
     const size_t per_block = static_cast<size_t>(kBlock) * cap * sizeof(uint2);
@@ -969,12 +999,15 @@ int cg_env_create(const cg_conf* conf, int device, cg_env** out) {
   if ((rc = dev_alloc(env, env->rgba, px * 4)) != CG_OK) return bail(rc);
   if ((rc = dev_alloc(env, env->stats, kStatBytes)) != CG_OK) return bail(rc);
   if ((rc = dev_alloc(env, env->queue, kXcds * kCounterStride * sizeof(uint32_t))) != CG_OK) return bail(rc);
+  if ((rc = dev_alloc(env, env->errflag, 256)) != CG_OK) return bail(rc);
+  if (const char* cap = std::getenv("CG_TEST_STACK_CAP")) env->test_stack_cap = std::max(0, std::atoi(cap));
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return bail(fail(env, CG_ERR_HIP, "hipGetDeviceProperties failed"));
   env->cu_count = prop.multiProcessorCount;
   if ((rc = scratch_resolve(env)) != CG_OK) return bail(rc);
   if (hipMemsetAsync(env->history.p, 0, env->history.bytes, env->stream) != hipSuccess ||
-      hipMemsetAsync(env->stats.p, 0, env->stats.bytes, env->stream) != hipSuccess)
+      hipMemsetAsync(env->stats.p, 0, env->stats.bytes, env->stream) != hipSuccess ||
+      hipMemsetAsync(env->errflag.p, 0, env->errflag.bytes, env->stream) != hipSuccess)
     return bail(fail(env, CG_ERR_HIP, "hipMemsetAsync failed"));
   WlTable wl;
   build_wl_table(wl);
@@ -989,7 +1022,8 @@ int cg_env_create(const cg_conf* conf, int device, cg_env** out) {
 
 int cg_env_destroy(cg_env* env) {
   if (!env) return fail(nullptr, CG_ERR_INVALID, "cg_env_destroy: null env");
-  if (env->stream) (void)hipStreamSynchronize(env->stream);
+  wait_all(env);  // no launch may still read what is freed below
+  while (!env->batches.empty()) cg_batch_destroy(env->batches.back());
   for (cg_scene* s : env->scenes) {
     release_scene(s);
     delete s;
@@ -1004,6 +1038,7 @@ int cg_env_destroy(cg_env* env) {
   dev_free(env->wl_leaf);
   dev_free(env->queue);
   dev_free(env->ovf);
+  dev_free(env->errflag);
   for (cg_vbatch* v : env->vbatch_all) {
     if (v->stream) (void)hipStreamSynchronize(v->stream);
     for (DevBuf* d : {&v->jobs, &v->first, &v->queue, &v->ovf}) dev_free(*d);
@@ -1022,7 +1057,7 @@ int cg_env_destroy(cg_env* env) {
 int cg_env_sync(cg_env* env) {
   if (!env) return fail(nullptr, CG_ERR_INVALID, "null env");
   CG_HIP(env, hipStreamSynchronize(env->stream));
-  return CG_OK;
+  return check_device_error(env);
 }
 
 void* cg_env_stream(cg_env* env) { return env ? static_cast<void*>(env->stream) : nullptr; }
@@ -1322,7 +1357,7 @@ int cg_scene_release(cg_env* env, cg_scene* scene) {
   if (!env || !scene) return fail(env, CG_ERR_INVALID, "cg_scene_release: null argument");
   auto it = std::find(env->scenes.begin(), env->scenes.end(), scene);
   if (it == env->scenes.end()) return fail(env, CG_ERR_INVALID, "scene does not belong to this env");
-  CG_HIP(env, hipStreamSynchronize(env->stream));
+  wait_all(env);  // view batches on their own streams may read the scene too
   release_scene(scene);
   delete scene;
   env->scenes.erase(it);
@@ -1519,7 +1554,7 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
   if ((rc = grow(env, v->jobs, n * sizeof(ViewArgs))) != CG_OK) return give_back(rc);
   if ((rc = grow(env, v->first, (n + 1) * sizeof(uint32_t))) != CG_OK) return give_back(rc);
   if ((rc = grow(env, v->queue, kXcds * kCounterStride * sizeof(uint32_t))) != CG_OK) return give_back(rc);
-  StackArgs sa{nullptr, 0};
+  StackArgs sa{nullptr, 0, static_cast<uint32_t*>(env->errflag.p)};
   if (depth > static_cast<uint32_t>(kShortStack)) {
     sa.ovf_cap = static_cast<int>(depth) - kShortStack;
     if ((rc = grow(env, v->ovf, static_cast<size_t>(grid) * kBlock * sa.ovf_cap * sizeof(uint2))) != CG_OK)
@@ -1589,7 +1624,7 @@ int cg_view_batch_query(cg_vbatch* v, int* done) {
 int cg_view_batch_wait(cg_vbatch* v) {
   if (!v) return fail(nullptr, CG_ERR_INVALID, "cg_view_batch_wait: null batch");
   CG_HIP(v->env, hipEventSynchronize(v->done));
-  return CG_OK;
+  return check_device_error(v->env);
 }
 
 int cg_view_batch_release(cg_vbatch* v) {
@@ -1605,7 +1640,7 @@ int cg_read_target(cg_env* env, cg_history_pixel* history, uint8_t* rgba8) {
   if (history) CG_HIP(env, hipMemcpyAsync(history, env->history.p, env->history.bytes, hipMemcpyDeviceToHost, env->stream));
   if (rgba8) CG_HIP(env, hipMemcpyAsync(rgba8, env->rgba.p, env->rgba.bytes, hipMemcpyDeviceToHost, env->stream));
   CG_HIP(env, hipStreamSynchronize(env->stream));
-  return CG_OK;
+  return check_device_error(env);
 }
 
 int cg_read_path_records(cg_env* env, uint32_t* path_hash, uint32_t* ray_count) {
@@ -1718,7 +1753,7 @@ struct cg_batch {
   void (*round)(cg_batch*, uint32_t) = nullptr;  // enqueue one round for the chosen kernel shape
   struct Job {
     bool active = false;
-    uint32_t slot_base = 0, nslots = 0;
+    uint32_t slot_base = 0, nslots = 0, scene = 0;
   };
   std::vector<Job> job;
   std::vector<WfJob> job_dev;        // host images of the device job table
@@ -1727,6 +1762,7 @@ struct cg_batch {
   uint32_t qlen_read[64];
   std::vector<const cg_scene*> scene_ptr;
   std::vector<DevScene> scene_dev;
+  std::vector<uint32_t> scene_jobs;  // active jobs per scene-table entry (0: the entry may be reused)
   std::vector<std::pair<uint32_t, uint32_t>> free_list;  // (first slot, count), sorted
   hipGraphExec_t exec = nullptr;
   uint64_t bytes = 0;
@@ -1868,6 +1904,7 @@ int cg_batch_create(cg_env* env, const cg_batch_conf* conf, cg_batch** out) {
   // baked into the graph), sized for the larger of the two resident grids.
   b->sa.ovf = nullptr;
   b->sa.ovf_cap = 0;
+  b->sa.err = static_cast<uint32_t*>(env->errflag.p);
   if (c.max_stack_depth > c.short_stack) {
     b->sa.ovf_cap = static_cast<int>(c.max_stack_depth - c.short_stack);
     alloc(b->ovf, static_cast<size_t>(std::max(b->grid_trace, b->grid_shade)) * c.block_size *
@@ -1903,14 +1940,17 @@ int cg_batch_create(cg_env* env, const cg_batch_conf* conf, cg_batch** out) {
   b->free_list.push_back({0u, c.slot_capacity});
   if (c.use_graph && (rc = batch_build_graph(b)) != CG_OK) return bail(rc);
   if (hipStreamSynchronize(env->stream) != hipSuccess) return bail(fail(env, CG_ERR_HIP, "stream sync failed"));
+  env->batches.push_back(b);
   *out = b;
   return CG_OK;
 }
 
 int cg_batch_destroy(cg_batch* b) {
   if (!b) return fail(nullptr, CG_ERR_INVALID, "null batch");
-  (void)hipStreamSynchronize(b->env->stream);
+  cg_env* env = b->env;
+  (void)hipStreamSynchronize(env->stream);
   batch_release(b);
+  env->batches.erase(std::remove(env->batches.begin(), env->batches.end(), b), env->batches.end());
   delete b;
   return CG_OK;
 }
@@ -1944,23 +1984,38 @@ int cg_batch_submit(cg_batch* b, const cg_job* job, uint32_t* job_id) {
   if (nslots > b->conf.slot_capacity)
     return fail(env, CG_ERR_INVALID, "job needs %llu slots; the batch has %u", static_cast<unsigned long long>(nslots),
                 b->conf.slot_capacity);
-  uint32_t sc = 0;
-  while (sc < b->scene_ptr.size() && b->scene_ptr[sc] != job->scene) ++sc;
-  if (sc == b->scene_ptr.size() && sc == kBatchMaxScenes) return fail(env, CG_ERR_FULL, "scene table full");
-  // (re)write the entry when new, or when a released scene's address came back
-  // for a new upload
-  if (sc == b->scene_ptr.size() || std::memcmp(&b->scene_dev[sc], &job->scene->view, sizeof(DevScene)) != 0) {
-    if (sc == b->scene_ptr.size()) b->scene_ptr.push_back(job->scene);
-    b->scene_dev[sc] = job->scene->view;
-    CG_HIP(env, hipMemcpyAsync(static_cast<DevScene*>(b->scenes.p) + sc, &b->scene_dev[sc], sizeof(DevScene),
-                               hipMemcpyHostToDevice, env->stream));
-  }
   uint32_t j = 0;
   while (j < b->job.size() && b->job[j].active) ++j;
   if (j == b->job.size()) return fail(env, CG_ERR_FULL, "job table full");
+  // Scene-table entry: the scene's own while a job of it is active (the same
+  // address and device view), else any entry no active job uses.  Entries
+  // free up as jobs finish, so a full table is "retry later".
+  uint32_t sc = 0;
+  while (sc < b->scene_ptr.size() &&
+         !(b->scene_ptr[sc] == job->scene && b->scene_jobs[sc] > 0 &&
+           std::memcmp(&b->scene_dev[sc], &job->scene->view, sizeof(DevScene)) == 0))
+    ++sc;
+  if (sc == b->scene_ptr.size()) {
+    sc = 0;
+    while (sc < b->scene_ptr.size() && b->scene_jobs[sc] > 0) ++sc;
+    if (sc == b->scene_ptr.size()) {
+      if (sc == kBatchMaxScenes) return fail(env, CG_ERR_FULL, "scene table full: %u scenes have active jobs", sc);
+      b->scene_ptr.push_back(nullptr);
+      b->scene_jobs.push_back(0u);
+    }
+  }
   uint32_t first = 0;
   if (!slots_alloc(b, static_cast<uint32_t>(nslots), first)) return fail(env, CG_ERR_FULL, "no room for %llu slots now",
                                                                          static_cast<unsigned long long>(nslots));
+  if (b->scene_jobs[sc] == 0u) {
+    b->scene_ptr[sc] = job->scene;
+    b->scene_dev[sc] = job->scene->view;
+    if (hipMemcpyAsync(static_cast<DevScene*>(b->scenes.p) + sc, &b->scene_dev[sc], sizeof(DevScene),
+                       hipMemcpyHostToDevice, env->stream) != hipSuccess) {
+      slots_free(b, first, static_cast<uint32_t>(nslots));
+      return fail(env, CG_ERR_HIP, "cg_batch_submit: scene table copy failed");
+    }
+  }
   WfJob& J = b->job_dev[j];
   J = WfJob{};
   J.scene = sc;
@@ -1995,6 +2050,8 @@ int cg_batch_submit(cg_batch* b, const cg_job* job, uint32_t* job_id) {
   b->job[j].active = true;
   b->job[j].slot_base = first;
   b->job[j].nslots = static_cast<uint32_t>(nslots);
+  b->job[j].scene = sc;
+  ++b->scene_jobs[sc];
   ++b->active;
   if (job_id) *job_id = j;
   return CG_OK;
@@ -2031,13 +2088,14 @@ int cg_batch_poll(cg_batch* b, uint32_t* done, uint32_t cap, uint32_t* n_done, u
     }
     b->job[j].active = false;
     slots_free(b, b->job[j].slot_base, b->job[j].nslots);
+    --b->scene_jobs[b->job[j].scene];
     --b->active;
   }
   if (n_done) *n_done = nd;
   if (n_active) *n_active = b->active;
   if (b->active > 0 && b->qlen_read[kWfQueueLen] == 0u && b->qlen_read[kWfQueueLen + 32u] == 0u)
     return fail(env, CG_ERR_STATE, "batch: %u jobs active but no rays queued", b->active);
-  return CG_OK;
+  return check_device_error(env);
 }
 
 int cg_batch_drain(cg_batch* b) {

@@ -120,6 +120,7 @@ struct Server::Impl {
   bool stop = false;
   std::string error;
   ServeStats stats;
+  uint64_t waiting = 0;  // submitted, not yet admitted (guarded by mu)
   std::atomic<uint64_t> next_id{1};
   std::thread worker;
 
@@ -140,6 +141,8 @@ struct Server::Impl {
     }
     cv_in.notify_all();
     if (worker.joinable()) worker.join();
+    for (Launch& L : launches) cg_view_batch_release(L.vb);  // waits: nothing in flight reads what is freed below
+    launches.clear();
     if (batch) cg_batch_destroy(batch);
     if (env) {
       if (arena_mem) cg_device_free(env, arena_mem);
@@ -153,6 +156,23 @@ struct Server::Impl {
     stop = true;
     cv_out.notify_all();
     return false;
+  }
+
+  // The worker ran into an error: every RenderAlone still waiting (in the
+  // worker's queue or not yet taken from the incoming one) gets it.
+  void fail_waiting_alone() {
+    std::lock_guard<std::mutex> g(mu);
+    for (std::deque<Request>* q : {&pending, &incoming})
+      for (Request& r : *q)
+        if (r.alone && r.done_alone) {
+          r.done_alone->set_value(unexpected<Error>(Error{error.empty() ? "Server: stopped" : error}));
+          r.done_alone = nullptr;
+        }
+  }
+
+  void run() {
+    loop();
+    fail_waiting_alone();
   }
 
   RenderResult finish(const Request& r, const std::vector<cg_history_pixel>& h) {
@@ -309,6 +329,7 @@ struct Server::Impl {
       launches.push_back(std::move(L));
       std::lock_guard<std::mutex> g(mu);
       stats.admitted += jobs.size();
+      waiting -= jobs.size();
     }
   }
 
@@ -367,6 +388,7 @@ struct Server::Impl {
         it = pending.erase(it);
         std::lock_guard<std::mutex> g(mu);
         ++stats.admitted;
+        --waiting;
       }
       if (active.empty()) continue;
       const auto t0 = Clock::now();
@@ -453,7 +475,7 @@ expected<std::unique_ptr<Server>, Error> Server::Create(const ServeConf& conf, c
     cg_env_set_counting(impl->env, 1);
   }
   Impl* p = impl.get();
-  impl->worker = std::thread([p] { p->loop(); });
+  impl->worker = std::thread([p] { p->run(); });
   return std::unique_ptr<Server>(new Server(std::move(impl)));
 }
 
@@ -479,8 +501,13 @@ expected<std::uint64_t, Error> Server::Submit(const RenderRequest& request) {
   {
     std::lock_guard<std::mutex> g(m.mu);
     if (!m.error.empty()) return unexpected<Error>(Error{m.error});
-    m.incoming.push_back(std::move(r));
     ++m.stats.submitted;
+    if (m.conf.max_waiting > 0 && m.waiting >= m.conf.max_waiting) {
+      ++m.stats.refused;
+      return unexpected<Error>(Error{"Submit: queue full (" + std::to_string(m.waiting) + " requests waiting)"});
+    }
+    m.incoming.push_back(std::move(r));
+    ++m.waiting;
   }
   m.cv_in.notify_all();
   return id;
@@ -551,13 +578,14 @@ typedef struct cgh_result_info {
 typedef struct cgh_serve_stats {
   uint64_t submitted, admitted, completed, rays;
   double gpu_busy_s;
+  uint64_t refused;
 } cgh_serve_stats;
 
 const char* cgh_serve_last_error(void) { return t_serve_err.c_str(); }
 
 int cgh_server_create(const cgh_scene* const* scenes, uint32_t n, uint32_t engine, uint64_t budget_bytes,
                       double output_fraction, uint32_t max_trace_depth, uint32_t in_flight, uint32_t rounds_per_launch,
-                      uint32_t block_size, uint32_t short_stack, cgh_server** out) {
+                      uint32_t block_size, uint32_t short_stack, uint32_t max_waiting, cgh_server** out) {
   if (!out || (n && !scenes)) return serve_fail("cgh_server_create: null argument");
   *out = nullptr;
   std::vector<const Scene*> sc;
@@ -571,6 +599,7 @@ int cgh_server_create(const cgh_scene* const* scenes, uint32_t n, uint32_t engin
   c.rounds_per_launch = rounds_per_launch;
   c.block_size = block_size;
   c.short_stack = short_stack;
+  c.max_waiting = max_waiting;
   auto s = Server::Create(c, sc);
   if (!s) return serve_fail(s.error());
   *out = reinterpret_cast<cgh_server*>(std::move(*s).release());
@@ -594,7 +623,11 @@ static RenderRequest make_request(uint32_t scene, const cg_camera* cam, const cg
 int cgh_server_submit(cgh_server* s, uint32_t scene, const cg_camera* cam, const cg_conf* conf, uint64_t* id) {
   if (!s || !cam || !conf || !id) return serve_fail("cgh_server_submit: null argument");
   auto r = reinterpret_cast<Server*>(s)->Submit(make_request(scene, cam, conf));
-  if (!r) return serve_fail(r.error());
+  if (!r) {
+    serve_fail(r.error());
+    // a refusal by the overload bound is "no room now", not a failure
+    return r.error().rfind("Submit: queue full", 0) == 0 ? CG_ERR_FULL : CG_ERR_INVALID;
+  }
   *id = *r;
   return CG_OK;
 }
@@ -640,6 +673,7 @@ int cgh_server_stats(cgh_server* s, cgh_serve_stats* out) {
   out->completed = st.completed;
   out->rays = st.rays;
   out->gpu_busy_s = st.gpu_busy_s;
+  out->refused = st.refused;
   return CG_OK;
 }
 

@@ -34,6 +34,9 @@ struct ServeConf {
   std::uint32_t rounds_per_launch = 16;                  // kWavefront
   std::uint32_t block_size = 0;   // kWavefront kernel shape, 0: default
   std::uint32_t short_stack = 0;
+  // Overload policy: Submit refuses a request (error "Submit: queue full")
+  // while this many requests wait for admission; 0 leaves the queue unbounded.
+  std::uint32_t max_waiting = 0;
 };
 
 struct RenderRequest {
@@ -53,6 +56,7 @@ struct RenderResult {
 
 struct ServeStats {
   std::uint64_t submitted = 0, admitted = 0, completed = 0;
+  std::uint64_t refused = 0;          // turned away by the max_waiting bound
   std::uint64_t rays = 0;             // closest-hit + shadow queries traced
   double gpu_busy_s = 0;              // time with at least one request in flight
 };
@@ -71,7 +75,8 @@ class Server {
   // output arena from the budget.
   static expected<std::unique_ptr<Server>, Error> Create(const ServeConf& conf, const std::vector<const Scene*>& scenes);
   ~Server();
-  // Thread-safe; the request arrives now.  Returns its id.
+  // Thread-safe; the request arrives now.  Returns its id, or the error
+  // "Submit: queue full" when max_waiting requests already wait.
   expected<std::uint64_t, Error> Submit(const RenderRequest& request);
   // The next finished request, waiting up to timeout_s; an empty result
   // (id 0) when none finished in time.

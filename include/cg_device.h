@@ -57,14 +57,18 @@ typedef struct cg_stats {
  * wgpu::CreateEnv (src/wgpu/environment.cpp:19-108); trace_depth is clamped
  * to [1, 64] as there (:20). */
 int cg_env_create(const cg_conf* conf, int device, cg_env** out);
-/* Release everything the env owns; replaces wgpu::Env::Term
- * (src/wgpu/environment.cpp:126-138). */
+/* Release everything the env owns, after waiting for all its streams (view
+ * batches included); wavefront batches still alive are destroyed with it.
+ * Replaces wgpu::Env::Term (src/wgpu/environment.cpp:126-138). */
 int cg_env_destroy(cg_env* env);
 /* Last error message of this env / of this thread ("" if none). */
 const char* cg_env_last_error(const cg_env* env);
 const char* cg_last_error(void);
 /* Wait for all enqueued work; replaces wgpu::Env::Poll
- * (src/wgpu/environment.cpp:110-116). */
+ * (src/wgpu/environment.cpp:110-116).  Reports CG_ERR_STATE if a traversal
+ * since the last check needed a deeper stack than the scene's bound (its
+ * walk dropped entries; the kernels still finish).  cg_read_target,
+ * cg_view_batch_wait and cg_batch_poll report it too; reporting clears it. */
 int cg_env_sync(cg_env* env);
 /* The env's hipStream_t, for callers that time or order work on it. */
 void* cg_env_stream(cg_env* env);
@@ -212,7 +216,8 @@ int cg_batch_destroy(cg_batch* batch);
 /* Device bytes held, and the largest run of free slots (what fits now). */
 int cg_batch_footprint(const cg_batch* batch, uint64_t* bytes, uint32_t* slots_free);
 /* Admit a job: CG_ERR_FULL when its (ceil(W/8)*ceil(H/8)*64) slots or a
- * table entry are not free now. */
+ * table entry are not free now (job entries and the 64 scene entries free up
+ * as jobs finish; a scene entry is reused once no active job reads it). */
 int cg_batch_submit(cg_batch* batch, const cg_job* job, uint32_t* job_id);
 /* Enqueue `launches` launches of rounds (asynchronous). */
 int cg_batch_launch(cg_batch* batch, uint32_t launches);

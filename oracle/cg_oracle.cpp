@@ -1149,7 +1149,10 @@ void camera_ray(const cg_camera& cam, uint32_t W, uint32_t H, uint32_t px, uint3
 // physics/srgb.wesl:5-27
 inline float encode_channel(float lin) {
   float v = fmx(lin, 0.0f);
-  float e = v <= 0.0031308f ? 12.92f * v : 1.055f * std::pow(v, 1.0f / 2.4f) - 0.055f;
+  // declared: pow(v, f32(1/2.4)) rounded once to binary32 from double precision
+  float e = v <= 0.0031308f ? 12.92f * v
+                            : 1.055f * static_cast<float>(std::pow(static_cast<double>(v),
+                                                                   static_cast<double>(1.0f / 2.4f))) - 0.055f;
   return clampf(e, 0.0f, 1.0f);
 }
 v3 xyz_to_srgb(v3 xyz) {

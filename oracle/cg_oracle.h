@@ -18,7 +18,9 @@
  *   - tan(acos(c))^2 in the GGX masking terms = max(0, 1 - c^2) / c^2;
  *   - sin/cos of the sampling angles: Cody-Waite reduction and Taylor
  *     polynomials (orc_sincos; about 1 ulp, inside WGSL's 2^-11 bound);
- *   - exp/acos/atan2/pow(x, 1/2.4) are the C library's;
+ *   - exp/acos/atan2 are the C library's; the sRGB power pow(x, f32(1/2.4))
+ *     is the double-precision power rounded once to binary32 (so the device,
+ *     which computes it the same way, matches it bit for bit);
  *   - material texture lookups: repeat-wrap in normalised coordinates, texel
  *     centres at (i + 1/2) / size, texels k / 255, and the blend
  *     lerp(lerp(T00, T10, a), lerp(T01, T11, a), b) with

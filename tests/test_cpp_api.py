@@ -53,6 +53,43 @@ def _std_supported(std: str) -> bool:
     return r.returncode == 0
 
 
+GLM_CALLER = r"""
+// A stand-in with glm's vector shape (glm itself is not installed here): the
+// reference's callers assign glm::vec3 / glm::vec2 into Camera and back.
+namespace glm {
+struct vec2 { float x, y; vec2(float a, float b) : x(a), y(b) {} };
+struct vec3 { float x, y, z; vec3(float a, float b, float c) : x(a), y(b), z(c) {} };
+}
+#include <CrystalGraphics/camera.h>
+#include <cstdio>
+
+int main() {
+  crystal::graphics::Camera cam;
+  cam.position = glm::vec3(1.0f, 2.0f, 3.0f);
+  cam.direction = glm::vec3{0.0f, 1.0f, 0.0f};
+  cam.viewport = glm::vec2(0.25f, 0.75f);
+  glm::vec3 back = cam.position;
+  glm::vec2 vp = cam.viewport;
+  crystal::graphics::Camera other{glm::vec3(4, 5, 6), {0, 0, 1}, {0.5f, 0.5f}};
+  static_assert(alignof(crystal::graphics::Camera) == 16 && sizeof(crystal::graphics::Camera) == 48);
+  std::printf("%g %g %g %g %g %g %g\n", back.x, back.y, back.z, vp.x, vp.y, other.position.z, cam.direction.y);
+  return 0;
+}
+"""
+
+
+@pytest.mark.parametrize("std", ["c++17", "c++20"])
+def test_camera_takes_glm_shaped_vectors(tmp_path, std):
+    src = tmp_path / "glm_caller.cpp"
+    src.write_text(GLM_CALLER)
+    exe = tmp_path / "glm_caller"
+    r = subprocess.run(["g++", f"-std={std}", "-Wall", "-Wextra", "-Werror", f"-I{ROOT / 'include'}", str(src),
+                        "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert out.stdout.split() == ["1", "2", "3", "0.25", "0.75", "6", "1"]
+
+
 @pytest.mark.parametrize("std", ["c++20", "c++23"])
 def test_api_program_compiles_and_links(tmp_path, std):
     if not _std_supported(std):

@@ -142,3 +142,35 @@ def test_submit_jobs_equal_single_renders(cg):
             ref = rj.read_history()["xyz"].reshape(conf.render_height, conf.render_width, 3)
             assert got.tobytes() == ref.tobytes()
             rj.free()
+
+
+def test_stack_overflow_is_reported(cg, monkeypatch):
+    """A traversal whose stack outgrows what the launch provides sets a device
+    error flag that cg_env_sync reports (and clears).  Forced from the host:
+    CG_TEST_STACK_CAP=0 gives the launch no overflow entries past the LDS part,
+    on a scene whose stack bound is deeper; pushes beyond are dropped, so every
+    kernel still finishes."""
+    s = cg.HostScene.generate("c0", seed=11)
+    sd = s.scene_data()
+    rays = s.rays(1 << 14, seed=3)
+    monkeypatch.setenv("CG_TEST_STACK_CAP", "0")
+    env = cg.DeviceEnv(cg.Conf.make(16, 16, 1, 1))
+    monkeypatch.delenv("CG_TEST_STACK_CAP")
+    try:
+        env.set_accel(0)
+        ds = env.upload(sd)
+        assert ds.footprint()[1] > 16
+        env.trace_closest(ds, rays)
+        with pytest.raises(cg.NativeError, match="stack overflow"):
+            env.sync()
+        env.sync()  # reported once, then clear
+    finally:
+        env.close()
+    ok = cg.DeviceEnv(cg.Conf.make(16, 16, 1, 1))
+    try:
+        ok.set_accel(0)
+        ds = ok.upload(sd)
+        ok.trace_closest(ds, rays)
+        ok.sync()  # the stack bound holds: no report
+    finally:
+        ok.close()

@@ -51,3 +51,27 @@ def test_serve_rejects_oversized_request(cg):
             srv.submit(0, scenes[0].describe()["camera"], cg.Conf.make(1920, 1080, 1, 2))
         with pytest.raises(cg.NativeError, match="max_trace_depth"):
             srv.submit(0, scenes[0].describe()["camera"], cg.Conf.make(16, 16, 1, 8))
+
+
+def test_serve_refuses_beyond_the_waiting_bound(cg):
+    """Overload policy: with max_waiting = 3, a burst of 40 requests submitted
+    at once is partly refused (submit returns None) and counted; every
+    accepted request completes and still equals its alone render."""
+    scenes = [cg.HostScene.generate("spheres", seed=4, params=[6])]
+    cam = scenes[0].describe()["camera"]
+    conf = cg.Conf.make(64, 48, 4, 6)
+    with cg.Server(scenes, budget_bytes=256 << 20, engine=0, max_trace_depth=8, in_flight=1, max_waiting=3) as srv:
+        accepted = [rid for rid in (srv.submit(0, cam, conf) for _ in range(40)) if rid is not None]
+        st = srv.stats()
+        assert st["refused"] > 0 and st["submitted"] == 40
+        assert len(accepted) + st["refused"] == 40
+        got = []
+        t0 = time.time()
+        while len(got) < len(accepted) and time.time() - t0 < 60:
+            r = srv.next(1.0)
+            if r is not None:
+                got.append(r)
+        assert sorted(r["id"] for r in got) == sorted(accepted)
+        ref = srv.render_alone(0, cam, conf)
+        assert all(r["history"].tobytes() == ref["history"].tobytes() for r in got)
+        assert srv.stats()["completed"] == len(accepted)
