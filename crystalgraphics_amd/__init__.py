@@ -289,6 +289,22 @@ class DeviceEnv:
     def stream_copy(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int) -> None:
         self._check(native.device().cg_stream_copy(self._h, dst.ptr, src.ptr, nbytes))
 
+    def alias_pick(self, scene: DeviceScene, u: np.ndarray) -> np.ndarray:
+        """The emitters the device's alias sampling picks for uniform pairs u (n x 2 float32)."""
+        u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+        n = u.shape[0]
+        ub = self.alloc(max(n, 1) * 8).upload(u)
+        ob = self.alloc(max(n, 1) * 4)
+        try:
+            self._check(native.device().cg_alias_pick(self._h, scene.handle, ub.ptr, n, ob.ptr))
+            return ob.download(np.uint32, n)
+        finally:
+            ub.free()
+            ob.free()
+
+    def stream_read(self, src: DeviceBuffer, nbytes: int, sink: DeviceBuffer) -> None:
+        self._check(native.device().cg_stream_read(self._h, src.ptr, nbytes, sink.ptr))
+
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
 

@@ -999,15 +999,22 @@ CG_HD float light_pdf(const DevScene& S, f3 next, f3 pos, const TStack& stk) {
 struct ESample {
   f3 dir;
   float prob, radiance, dist;
+  uint32_t prim;  // the emitter the alias table picked
 };
+// sampleEmissivePrim (pdf/emission.wesl:181-191) for two uniforms: bin =
+// u32(u1 * n), which is n when u1 == 1.0; the table read and the emitter it
+// names are clamped to the arrays as WebGPU robustness clamps them.
+CG_HD uint32_t alias_pick(const DevScene& S, float u1, float u2) {
+  const uint32_t n = S.alias_count;
+  const uint32_t bin = f2u(u1 * static_cast<float>(n));
+  const float2 ent = S.alias[umin(bin, n - 1u)];
+  return umin(u2 < ent.y ? bin : __builtin_bit_cast(uint32_t, ent.x), S.emit_count - 1u);
+}
 CG_HD ESample emission_sample(const DevScene& S, const Surface& s, const SmitsBin& b, uint32_t& rng) {
-  uint32_t n = S.alias_count;
-  uint32_t bin = f2u(rng_float(rng) * static_cast<float>(n));
-  float randf = rng_float(rng);
-  uint32_t cb = umin(bin, n - 1u);
-  float2 ent = S.alias[cb];
-  uint32_t pick = randf < ent.y ? bin : __builtin_bit_cast(uint32_t, ent.x);
-  const EmitRec& e = S.emit[umin(pick, S.emit_count - 1u)];
+  const float u1 = rng_float(rng);
+  const float u2 = rng_float(rng);
+  const uint32_t pick = alias_pick(S, u1, u2);
+  const EmitRec& e = S.emit[pick];
   float tx = rng_float(rng);
   float ty = rng_float(rng);
   float sq = CG_SQRT(tx);
@@ -1019,7 +1026,7 @@ CG_HD ESample emission_sample(const DevScene& S, const Surface& s, const SmitsBi
   f3 ln = mk3(e.ln_x, e.ln_y, e.ln_z);
   float lp = CG_FABS(dot(ln, -next));
   float sapdf = e.area_pdf * dsq / fmx(lp, 1e-3f);
-  return ESample{next, sapdf, rgb_radiance(ld3(e.rgb), b), CG_SQRT(dsq)};
+  return ESample{next, sapdf, rgb_radiance(ld3(e.rgb), b), CG_SQRT(dsq), pick};
 }
 
 // ---------------------------------------------------------------------------
@@ -1088,7 +1095,9 @@ struct Frame5 {
   float tcoeff[4];
   f3 edir;
   float erad, edist, emission, acc;
-  uint32_t tcount, ecount, iter;
+  uint32_t tcount;
+  uint32_t ecount;  // 0: no light sample; else 1 + the emitter it picked
+  uint32_t iter;
 };
 CG_HD void forward_pass(const DevScene& S, f3 d, float l, const SmitsBin& b, const Surface& s, int depth,
                         const TraceParams& P, uint32_t& rng, Frame5& x, const TStack& stk) {
@@ -1140,7 +1149,7 @@ CG_HD void forward_pass(const DevScene& S, f3 d, float l, const SmitsBin& b, con
     x.edir = e.dir;
     x.erad = e.radiance * coeff * proj / balance_den(S, d, e.dir, s, m, c, kNone, -1.0f, e.prob, stk);
     x.edist = e.dist;
-    x.ecount++;
+    x.ecount = 1u + e.prim;  // one light sample at most (MAX_EMISSION_SAMPLE_COUNT)
   }
   x.emission = m.emission;
   x.pos = s.pos;

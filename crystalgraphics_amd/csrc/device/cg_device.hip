@@ -460,6 +460,44 @@ __global__ __launch_bounds__(kBlock) void k_stream_copy(float4* __restrict__ dst
   for (size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// Alias-table picks for given uniform pairs (the check of the emitter
+// indices next-event estimation draws).
+__global__ __launch_bounds__(kBlock) void k_alias_pick(DevScene S, const float2* __restrict__ u, uint32_t n,
+                                                       uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    const float2 v = u[i];
+    out[i] = alias_pick(S, v.x, v.y);
+  }
+}
+
+// Read-only probe: every 16-byte vector of src is loaded once (four loads in
+// flight per lane per step), folded into a per-lane XOR and one word per
+// block is written, so the kernel's traffic is the read stream alone.
+__global__ __launch_bounds__(kBlock) void k_stream_read(const uint4* __restrict__ src, size_t n,
+                                                        uint32_t* __restrict__ sink) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * kBlock;
+  size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x;
+  uint32_t acc = 0u;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+    acc ^= (a.x ^ a.y ^ a.z ^ a.w) ^ (b.x ^ b.y ^ b.z ^ b.w) ^ (c.x ^ c.y ^ c.z ^ c.w) ^ (d.x ^ d.y ^ d.z ^ d.w);
+  }
+  for (; i < n; i += stride) {
+    const uint4 a = src[i];
+    acc ^= a.x ^ a.y ^ a.z ^ a.w;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc ^= __shfl_xor(acc, off, 64);
+  __shared__ uint32_t s_acc[kBlock / 64];
+  if ((threadIdx.x & 63u) == 0u) s_acc[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t v = 0u;
+    for (int w = 0; w < kBlock / 64; ++w) v ^= s_acc[w];
+    sink[blockIdx.x] = v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Wavefront rounds (cg_wavefront.h).  Queues hold slot ids; a wave appends
 // the slots of its lanes that continue with one atomic (ballot, popcount,
@@ -871,6 +909,7 @@ int launch_view(cg_env* env, const ViewArgs& a, int frames, uint32_t tiles, uint
 // first kScratchQueueBytes, then the overflow traversal stack.  Every device
 // the library runs on holds one copy for the life of the process; host code
 // uses only the per-device address that hipGetSymbolAddress returns.
+constexpr uint32_t kStreamReadBlocks = 256u * 16u;  // grid of the read-only probe (16 blocks per CU)
 constexpr size_t kScratchQueueBytes = 4096;
 static_assert(kXcds * kCounterStride * sizeof(uint32_t) <= kScratchQueueBytes, "queue counters fit the scratch head");
 constexpr int kMaxDevices = 64;
@@ -1702,6 +1741,29 @@ int cg_stream_copy(cg_env* env, void* dst, const void* src, size_t bytes) {
   const size_t blocks = std::min<size_t>((n + kBlock - 1) / kBlock, 256u * 64u);
   hipLaunchKernelGGL(k_stream_copy, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, env->stream,
                      static_cast<float4*>(dst), static_cast<const float4*>(src), n);
+  CG_HIP(env, hipGetLastError());
+  return CG_OK;
+}
+
+int cg_alias_pick(cg_env* env, cg_scene* scene, const float* u, uint32_t n, uint32_t* out) {
+  if (!env || !scene || (n && (!u || !out))) return fail(env, CG_ERR_INVALID, "cg_alias_pick: null argument");
+  if (scene->view.alias_count == 0 || scene->view.emit_count == 0)
+    return fail(env, CG_ERR_STATE, "cg_alias_pick: the scene has no emitters");
+  if (n == 0) return CG_OK;
+  hipLaunchKernelGGL(k_alias_pick, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, env->stream, scene->view,
+                     reinterpret_cast<const float2*>(u), n, out);
+  CG_HIP(env, hipGetLastError());
+  return CG_OK;
+}
+
+int cg_stream_read(cg_env* env, const void* src, size_t bytes, uint32_t* sink) {
+  if (!env || !src || !sink) return fail(env, CG_ERR_INVALID, "cg_stream_read: null argument");
+  if (bytes % 16 != 0) return fail(env, CG_ERR_INVALID, "cg_stream_read: size must be a multiple of 16");
+  const size_t n = bytes / 16;
+  if (n == 0) return CG_OK;
+  const uint32_t blocks = static_cast<uint32_t>(std::min<size_t>((n + kBlock - 1) / kBlock, kStreamReadBlocks));
+  hipLaunchKernelGGL(k_stream_read, dim3(blocks), dim3(kBlock), 0, env->stream, static_cast<const uint4*>(src), n,
+                     sink);
   CG_HIP(env, hipGetLastError());
   return CG_OK;
 }

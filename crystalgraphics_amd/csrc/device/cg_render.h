@@ -79,7 +79,7 @@ CG_HD void render_pixel(const ViewArgs& A, uint32_t px, uint32_t py, const TStac
           HitRec sh = traverse<true>(S, so, f.edir, fmn(expected - 1e-3f, kMax), stk, st);
           ++nshadow;
           bool occ = sh.inst != CG_MISS_ID;
-          hash = hmix(hash, 0xA5000000u | (occ ? 1u : 0u));
+          hash = hmix(hmix(hash, 0xA5000000u | (occ ? 1u : 0u)), f.ecount - 1u);  // outcome, emitter
           if (occ) f.erad = 0.0f;
         }
         if (f.tcount != 0u) {

@@ -189,7 +189,7 @@ CG_HD bool wf_step(const WfTables& T, const WfJob& J, const DevScene& S, uint32_
     // solveEmission's visibility test (trace.wesl:91-113)
     Frame5 f = F[s.depth * fs];
     const bool occ = hw.inst != CG_MISS_ID;
-    s.hash = hmix(s.hash, 0xA5000000u | (occ ? 1u : 0u));
+    s.hash = hmix(hmix(s.hash, 0xA5000000u | (occ ? 1u : 0u)), f.ecount - 1u);  // outcome, emitter
     if (occ) f.erad = 0.0f;
     s.shadow = false;
     if (f.tcount != 0u) {

@@ -257,6 +257,8 @@ def device() -> C.CDLL:
         _sig(d, "cg_trace_closest_noalloc", I32, P, P, P, U32, P, P)
         _sig(d, "cg_trace_occluded", I32, P, P, P, P, U32, P)
         _sig(d, "cg_stream_copy", I32, P, P, P, C.c_size_t)
+        _sig(d, "cg_stream_read", I32, P, P, C.c_size_t, P)
+        _sig(d, "cg_alias_pick", I32, P, P, P, U32, P)
         _sig(d, "cg_device_alloc", I32, P, C.c_size_t, C.POINTER(P))
         _sig(d, "cg_device_free", I32, P, P)
         _sig(d, "cg_memcpy_h2d", I32, P, P, P, C.c_size_t)

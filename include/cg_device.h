@@ -73,7 +73,9 @@ int cg_env_sync(cg_env* env);
 /* The env's hipStream_t, for callers that time or order work on it. */
 void* cg_env_stream(cg_env* env);
 /* Enable (1) or disable (0) traversal counting and, for cg_view, per-pixel
- * path records (hash of the hit-id sequence and query count). */
+ * path records (hash of the pixel's discrete decisions -- hit ids, shadow
+ * outcomes with the emitter each light sample picked, sky texels -- and its
+ * query count). */
 int cg_env_set_counting(cg_env* env, int enable);
 int cg_env_set_path_records(cg_env* env, int enable);
 int cg_stats_read(cg_env* env, cg_stats* out);  /* synchronises */
@@ -147,11 +149,25 @@ int cg_trace_closest_noalloc(cg_env* env, cg_scene* scene, const cg_ray* rays, u
 int cg_trace_occluded(cg_env* env, cg_scene* scene, const cg_ray* rays, const float* tmax, uint32_t n,
                       uint32_t* occluded);
 
+/* The emitter next-event estimation picks for given uniforms (u: 2n floats,
+ * pairs (u1, u2); out: n indices; device buffers): the reference's
+ * sampleEmissivePrim (shader/compute/shading/pdf/emission.wesl:181-191) on
+ * the scene's uploaded alias table, with the same device code the renders
+ * use.  bin = u32(u1 * N) reaches N at u1 = 1.0; reads and the returned
+ * emitter are clamped to the tables as WebGPU robustness clamps them. */
+int cg_alias_pick(cg_env* env, cg_scene* scene, const float* u, uint32_t n, uint32_t* out);
+
 /* Measurement probe (no reference counterpart): copy `bytes` (a multiple of
  * 16) from src to dst, both device buffers, with a plain coalesced 16-byte
  * vector copy on the env's stream.  Its rate is the achievable HBM
  * bandwidth, and its counters the scale for hardware byte counters. */
 int cg_stream_copy(cg_env* env, void* dst, const void* src, size_t bytes);
+/* Measurement probe (no reference counterpart): read `bytes` (a multiple of
+ * 16) of the device buffer src once, four 16-byte loads in flight per lane,
+ * writing one XOR word per workgroup to the device buffer sink (at least
+ * 4096 words).  Its rate is the achievable read-only HBM bandwidth, the
+ * denominator for read-dominated kernels such as traversal. */
+int cg_stream_read(cg_env* env, const void* src, size_t bytes, uint32_t* sink);
 
 /* HBM buffers for the batch entry points. */
 int cg_device_alloc(cg_env* env, size_t bytes, void** dptr);

@@ -874,6 +874,7 @@ Hit ray_hit(const orc_scene& sc, v3 pos, v3 dir, Counts& cnt, bool cull, float* 
 struct ESample {
   v3 dir;
   float prob, radiance, expected_dist;
+  uint32_t prim;  // the emitter the alias table picked
 };
 inline bool can_sample(const orc_scene& sc) { return !sc.alias.empty(); }
 uint32_t sample_emissive_prim(const orc_scene& sc, uint32_t& rng) {  // emission.wesl:181-191
@@ -885,8 +886,9 @@ uint32_t sample_emissive_prim(const orc_scene& sc, uint32_t& rng) {  // emission
   return std::min<uint32_t>(pick, static_cast<uint32_t>(sc.em.size()) - 1u);
 }
 ESample emission_sample(const orc_scene& sc, float l, const Surface& s, uint32_t& rng) {  // :46-96
-  if (!can_sample(sc)) return ESample{mk(0, 0, 0), 0.0f, 0.0f, 0.0f};
-  const cg_emissive_prim& p = sc.em[sample_emissive_prim(sc, rng)];
+  if (!can_sample(sc)) return ESample{mk(0, 0, 0), 0.0f, 0.0f, 0.0f, 0u};
+  const uint32_t prim = sample_emissive_prim(sc, rng);
+  const cg_emissive_prim& p = sc.em[prim];
   float tx = rand_float(rng);
   float ty = rand_float(rng);
   float sq = std::sqrt(tx);
@@ -900,7 +902,7 @@ ESample emission_sample(const orc_scene& sc, float l, const Surface& s, uint32_t
   float lp = std::fabs(dot(ln, -next));
   float apdf = fmx(p.weight / p.area, 1e-3f);
   float sapdf = apdf * dsq / fmx(lp, 1e-3f);
-  return ESample{next, sapdf, rgb_to_radiance(mk(p.emission), l), std::sqrt(dsq)};
+  return ESample{next, sapdf, rgb_to_radiance(mk(p.emission), l), std::sqrt(dsq), prim};
 }
 float triangle_hit_dist(v3 pos, v3 dir, const cg_emissive_prim& p) {  // :151-179
   v3 p0 = mk(p.v[0]), p1 = mk(p.v[1]), p2 = mk(p.v[2]);
@@ -1012,6 +1014,7 @@ struct Ctx {  // shading.wesl:24-28 + sample.wesl:73-78
   float tcoeff[4];
   v3 edir;
   float erad = 0, edist = 0;
+  uint32_t eprim = 0;
   float emission = 0;
 };
 Ctx forward_pass(const orc_scene& sc, v3 d, float l, const Surface& s, int depth, const Params& P,
@@ -1056,6 +1059,7 @@ Ctx forward_pass(const orc_scene& sc, v3 d, float l, const Surface& s, int depth
     x.edir = e.dir;
     x.erad = e.radiance * coeff * proj / balance_denominator(sc, d, e.dir, s, m, c, M_NONE, -1.0f, e.prob);
     x.edist = e.expected_dist;
+    x.eprim = e.prim;
     x.ecount++;
   }
   x.emission = m.emission;
@@ -1118,7 +1122,8 @@ struct Tracer {
       Hit sh = ray_hit(sc, sp, sd, cnt, cull, nullptr, nullptr);
       rays++;
       bool occluded = sh.hit && sh.dist < (expected - 1e-3f);
-      hash = hmix(hash, 0xA5000000u | (occluded ? 1u : 0u));
+      // the shadow outcome and the emitter the alias table picked are discrete decisions
+      hash = hmix(hmix(hash, 0xA5000000u | (occluded ? 1u : 0u)), x.eprim);
       if (occluded) x.erad = 0.0f;
     }
     float tr[4] = {0, 0, 0, 0};
@@ -1889,12 +1894,9 @@ float orc_light_pdf(const orc_scene* sc, const float* pos, const float* next_dir
 
 float orc_emission_sample(const orc_scene* sc, float lambda, const float* pos, uint32_t* rng_state, float* next_dir,
                           float* expected_dist, float* radiance, uint32_t* prim) {
-  if (prim) {
-    uint32_t probe = *rng_state;
-    *prim = can_sample(*sc) ? sample_emissive_prim(*sc, probe) : 0xFFFFFFFFu;
-  }
   const float up[3] = {0.0f, 0.0f, 1.0f};  // emissionSample reads only the surface position
   ESample e = emission_sample(*sc, lambda, unit_surface(pos, up), *rng_state);
+  if (prim) *prim = can_sample(*sc) ? e.prim : 0xFFFFFFFFu;
   put3(next_dir, e.dir);
   *expected_dist = e.expected_dist;
   *radiance = e.radiance;

@@ -66,7 +66,8 @@ void orc_trace_closest(const orc_scene* s, const cg_ray* rays, uint32_t n,
 /* The compute pass over pixel rows [y0, y1) (compute.wesl:32-62).  history is
  * the full W*H buffer, read when iter_count > 0 and written for the rows.
  * rgba8 (nullable, W*H*4), path_hash (nullable, W*H) receives a hash of the
- * pixel's sequence of hit ids, shadow outcomes and sky texels, ray_count
+ * pixel's sequence of hit ids, shadow outcomes (each with the emitter its
+ * light sample picked) and sky texels, ray_count
  * (nullable, W*H)
  * its closest-hit queries.  Returns 0, or -1 on an unsupported scene. */
 int orc_render(const orc_scene* s, const cg_conf* conf, const cg_camera* cam,
