@@ -243,6 +243,12 @@ class DeviceEnv:
         built on the device)."""
         self._check(native.device().cg_env_set_accel(self._h, int(mode)))
 
+    RNG_REFERENCE, RNG_SAMPLE = 0, 1
+
+    def set_rng_mode(self, mode: int) -> None:
+        """RNG_REFERENCE: the reference's one stream per pixel; RNG_SAMPLE: one stream per sample (cg_device.h)."""
+        self._check(native.device().cg_env_set_rng_mode(self._h, mode))
+
     def set_counting(self, on: bool = True) -> None:
         self._check(native.device().cg_env_set_counting(self._h, int(on)))
 
@@ -373,8 +379,9 @@ class RenderJob:
     target and path records."""
 
     def __init__(self, env: DeviceEnv, scene: DeviceScene, camera: Camera, conf: Conf, iter_count: int = 0,
-                 path_records: bool = False, history: np.ndarray | None = None):
+                 path_records: bool = False, history: np.ndarray | None = None, rng_mode: int = 0):
         self.env, self.scene, self.camera, self.conf, self.iter_count = env, scene, camera, conf, iter_count
+        self.rng_mode = rng_mode
         n = conf.render_width * conf.render_height
         self.npix = n
         self.history = env.alloc(n * HISTORY.itemsize)
@@ -391,6 +398,7 @@ class RenderJob:
         j.camera = self.camera
         j.conf = self.conf
         j.iter_count = self.iter_count
+        j.rng_mode = self.rng_mode
         j.history = self.history.ptr
         j.rgba8 = self.rgba.ptr
         j.path_hash = self.path_hash.ptr if self.path_hash else None

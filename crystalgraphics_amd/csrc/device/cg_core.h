@@ -112,6 +112,24 @@ CG_HD float rng_float(uint32_t& s) {
   rng_step(s);
   return static_cast<float>(s) / 4294967296.0f;
 }
+// Sample-stream mode (declared for this build; the reference has one stream
+// per pixel): sample i of a pixel draws from the stream seeded
+// step(step(b) ^ (i + 1) * 0x9E3779B9) from the pixel's base seed b, and the
+// pixel carries step(step(b)) to the next frame (the "i = -1" stream; step is
+// a bijection, so it differs from every sample's).
+CG_HD uint32_t sample_seed(uint32_t base, uint32_t i) {
+  uint32_t t = base;
+  rng_step(t);
+  t ^= (i + 1u) * 0x9E3779B9u;
+  rng_step(t);
+  return t;
+}
+CG_HD uint32_t sample_carry(uint32_t base) {
+  uint32_t t = base;
+  rng_step(t);
+  rng_step(t);
+  return t;
+}
 
 constexpr float kPi = 3.1415926535f;
 constexpr float kMax = 1e6f;

@@ -85,6 +85,8 @@ struct cg_env {
   std::vector<cg_vbatch*> vbatch_all;
   void* scratch = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress)
   DevBuf errflag;           // device error flag: a traversal stack overflowed
+  DevBuf samples;           // sample-stream mode: per-sample results of the last render
+  uint32_t rng_mode = CG_RNG_REFERENCE;
   int test_stack_cap = -1;  // CG_TEST_STACK_CAP: caps overflow entries (tests force an overflow)
   std::vector<cg_batch*> batches;  // live wavefront batches, destroyed with the env
 };
@@ -378,6 +380,37 @@ __global__ __launch_bounds__(kBlock, 3) void k_view(ViewArgs A, WorkQueue q, Sta
     if (px < A.W && py < A.H) render_pixel<MF>(A, px, py, stk, st, nrays, nshadow);
   }
   if (A.stats) flush_stats(A.stats, nrays, nshadow, st.nodes, st.tris, st.insts);
+}
+
+// Sample-stream mode: the work items are (8x8 tile, chunk of `chunk`
+// samples) pairs, a tile's chunks adjacent, so one pixel's samples run on
+// several waves at once and no wave carries a long chain of dependent
+// queries; the per-sample results go to B and k_view_finish sums them.
+template <int MF>
+__global__ __launch_bounds__(kBlock, 4) void k_view_s(ViewArgs A, SampleBufs B, uint32_t chunk, uint32_t nchunks,
+                                                      WorkQueue q, StackArgs sa) {
+  __shared__ uint32_t s_ref[kShortStack * kBlock];
+  __shared__ float s_t[kShortStack * kBlock];
+  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kShortStack);
+  const uint32_t tiles_x = (A.W + 7u) / 8u;
+  TravStats st{0u, 0u, 0u};
+  uint32_t nrays = 0, nshadow = 0;
+  uint32_t probe = 0;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (;;) {
+    const uint32_t item = grab_item(q, probe);
+    if (item == kNoItem) break;
+    const uint32_t tile = item / nchunks, c = item - tile * nchunks;
+    const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u);
+    const uint32_t py = (tile / tiles_x) * 8u + (lane >> 3);
+    const uint32_t s0 = c * chunk, s1 = s0 + chunk < A.spp ? s0 + chunk : A.spp;
+    if (px < A.W && py < A.H) render_samples<MF>(A, B, px, py, s0, s1, stk, st, nrays, nshadow);
+  }
+  if (A.stats) flush_stats(A.stats, nrays, nshadow, st.nodes, st.tris, st.insts);
+}
+__global__ __launch_bounds__(kBlock) void k_view_finish(ViewArgs A, SampleBufs B) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < A.W * A.H) finish_pixel(A, B, i);
 }
 
 // Many render jobs in one persistent launch of the same per-pixel kernel:
@@ -898,11 +931,56 @@ int launch_view_mf(cg_env* env, const ViewArgs& a, uint32_t tiles, uint32_t dept
   hipLaunchKernelGGL((k_view<MF>), dim3(grid), dim3(kBlock), 0, env->stream, a, q, sa);
   return CG_OK;
 }
-int launch_view(cg_env* env, const ViewArgs& a, int frames, uint32_t tiles, uint32_t depth) {
+int launch_view_ref(cg_env* env, const ViewArgs& a, int frames, uint32_t tiles, uint32_t depth) {
   if (frames <= 4) return launch_view_mf<4>(env, a, tiles, depth);
   if (frames <= 8) return launch_view_mf<8>(env, a, tiles, depth);
   if (frames <= 16) return launch_view_mf<16>(env, a, tiles, depth);
   return launch_view_mf<64>(env, a, tiles, depth);
+}
+// Samples per work item of the sample-stream mode: one sample per item up to
+// 16 samples per pixel, then enough that a tile has at most 16 items (the
+// image does not depend on it: samples are summed in sample order).
+uint32_t sample_chunk(uint32_t spp) { return spp <= 16u ? 1u : (spp + 15u) / 16u; }
+template <int MF>
+int launch_view_s_mf(cg_env* env, const ViewArgs& a, const SampleBufs& b, uint32_t tiles, uint32_t depth) {
+  const uint32_t chunk = sample_chunk(a.spp), nchunks = (a.spp + chunk - 1u) / chunk;
+  const uint64_t items = static_cast<uint64_t>(tiles) * nchunks;
+  if (items >= kNoItem) return fail(env, CG_ERR_INVALID, "render too large for the sample-stream mode");
+  const uint32_t grid = resident_grid(k_view_s<MF>, env, static_cast<uint32_t>((items + 3u) / 4u));
+  StackArgs sa;
+  int rc = stack_args(env, grid, depth, sa, kShortStack);
+  if (rc != CG_OK) return rc;
+  WorkQueue q = reset_queue(env, static_cast<uint32_t>(items));
+  hipLaunchKernelGGL((k_view_s<MF>), dim3(grid), dim3(kBlock), 0, env->stream, a, b, chunk, nchunks, q, sa);
+  const uint32_t npix = a.W * a.H;
+  hipLaunchKernelGGL(k_view_finish, dim3((npix + kBlock - 1u) / kBlock), dim3(kBlock), 0, env->stream, a, b);
+  return CG_OK;
+}
+// The sample-stream buffers of a render: W*H*spp XYZ values (and path records).
+int sample_bufs(cg_env* env, const ViewArgs& a, SampleBufs& b) {
+  const size_t n = static_cast<size_t>(a.W) * a.H * a.spp;
+  const size_t bytes = n * sizeof(float4) + (a.path_hash ? n * sizeof(uint2) : 0);
+  if (env->samples.bytes < bytes) {
+    if (env->samples.p) {
+      CG_HIP(env, hipStreamSynchronize(env->stream));
+      dev_free(env->samples);
+    }
+    int rc = dev_alloc(env, env->samples, bytes);
+    if (rc != CG_OK) return rc;
+  }
+  b.xyz = static_cast<float4*>(env->samples.p);
+  b.records = a.path_hash ? reinterpret_cast<uint2*>(static_cast<char*>(env->samples.p) + n * sizeof(float4)) : nullptr;
+  return CG_OK;
+}
+int launch_view(cg_env* env, const ViewArgs& a, int frames, uint32_t tiles, uint32_t depth, uint32_t rng_mode) {
+  if (rng_mode == CG_RNG_REFERENCE || a.spp == 0u) return launch_view_ref(env, a, frames, tiles, depth);
+  SampleBufs b{};
+  int rc = sample_bufs(env, a, b);
+  if (rc != CG_OK) return rc;
+  if (frames <= 4) return launch_view_s_mf<4>(env, a, b, tiles, depth);
+  if (frames <= 8) return launch_view_s_mf<8>(env, a, b, tiles, depth);
+  if (frames <= 16) return launch_view_s_mf<16>(env, a, b, tiles, depth);
+  return launch_view_s_mf<64>(env, a, b, tiles, depth);
 }
 // ---------------------------------------------------------------------------
 // Fixed scratch of cg_trace_closest_noalloc: the work-queue counters in its
@@ -1078,6 +1156,7 @@ int cg_env_destroy(cg_env* env) {
   dev_free(env->queue);
   dev_free(env->ovf);
   dev_free(env->errflag);
+  dev_free(env->samples);
   for (cg_vbatch* v : env->vbatch_all) {
     if (v->stream) (void)hipStreamSynchronize(v->stream);
     for (DevBuf* d : {&v->jobs, &v->first, &v->queue, &v->ovf}) dev_free(*d);
@@ -1117,6 +1196,13 @@ int cg_env_set_path_records(cg_env* env, int enable) {
     rc = dev_alloc(env, env->ray_count, px * 4);
     if (rc != CG_OK) return rc;
   }
+  return CG_OK;
+}
+
+int cg_env_set_rng_mode(cg_env* env, int mode) {
+  if (!env) return fail(nullptr, CG_ERR_INVALID, "null env");
+  if (mode != CG_RNG_REFERENCE && mode != CG_RNG_SAMPLE) return fail(env, CG_ERR_INVALID, "unknown rng mode %d", mode);
+  env->rng_mode = static_cast<uint32_t>(mode);
   return CG_OK;
 }
 
@@ -1439,7 +1525,7 @@ int cg_view(cg_env* env, cg_scene* scene, const cg_camera* camera, uint32_t iter
   a.stats = env->counting ? static_cast<unsigned long long*>(env->stats.p) : nullptr;
   const uint32_t tiles = ((a.W + 7u) / 8u) * ((a.H + 7u) / 8u);
   const int frames = std::max(1, a.P.D - 1);
-  int rc = launch_view(env, a, frames, tiles, scene->stack_depth);
+  int rc = launch_view(env, a, frames, tiles, scene->stack_depth, env->rng_mode);
   if (rc != CG_OK) return rc;
   CG_HIP(env, hipGetLastError());
   return CG_OK;
@@ -1478,7 +1564,9 @@ int cg_view_job(cg_env* env, const cg_job* job) {
   a.stats = env->counting ? static_cast<unsigned long long*>(env->stats.p) : nullptr;
   if (a.path_hash && !a.ray_count) return fail(env, CG_ERR_INVALID, "cg_view_job: path_hash needs ray_count");
   const uint32_t tiles = ((a.W + 7u) / 8u) * ((a.H + 7u) / 8u);
-  int rc = launch_view(env, a, std::max(1, a.P.D - 1), tiles, job->scene->stack_depth);
+  if (job->rng_mode != CG_RNG_REFERENCE && job->rng_mode != CG_RNG_SAMPLE)
+    return fail(env, CG_ERR_INVALID, "cg_view_job: unknown rng_mode %u", job->rng_mode);
+  int rc = launch_view(env, a, std::max(1, a.P.D - 1), tiles, job->scene->stack_depth, job->rng_mode);
   if (rc != CG_OK) return rc;
   CG_HIP(env, hipGetLastError());
   return CG_OK;

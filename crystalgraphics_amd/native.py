@@ -136,7 +136,7 @@ class BatchConf(C.Structure):
 class Job(C.Structure):
     """cg_job"""
     _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("conf", Conf), ("iter_count", C.c_uint32),
-                ("_pad", C.c_uint32), ("history", C.c_void_p), ("rgba8", C.c_void_p), ("path_hash", C.c_void_p),
+                ("rng_mode", C.c_uint32), ("history", C.c_void_p), ("rgba8", C.c_void_p), ("path_hash", C.c_void_p),
                 ("ray_count", C.c_void_p)]
 
 
@@ -243,6 +243,7 @@ def device() -> C.CDLL:
         _sig(d, "cg_env_set_path_records", I32, P, I32)
         _sig(d, "cg_stats_read", I32, P, C.POINTER(Stats))
         _sig(d, "cg_env_set_accel", I32, P, I32)
+        _sig(d, "cg_env_set_rng_mode", I32, P, I32)
         _sig(d, "cg_stats_reset", I32, P)
         _sig(d, "cg_scene_upload", I32, P, C.POINTER(SceneDataC), C.POINTER(P))
         _sig(d, "cg_scene_refit", I32, P, P, C.POINTER(SceneDataC))

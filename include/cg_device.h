@@ -90,6 +90,21 @@ int cg_stats_read(cg_env* env, cg_stats* out);  /* synchronises */
  * All give the same closest hits (up to triangles tied at the same t). */
 enum { CG_ACCEL_REFERENCE = 0, CG_ACCEL_REBUILT = 1, CG_ACCEL_DEVICE = 2 };
 int cg_env_set_accel(cg_env* env, int mode);
+/* Random-number streams of cg_view (and of cg_view_job through
+ * cg_job.rng_mode):
+ *   CG_RNG_REFERENCE (default): one stream per pixel consumed in the
+ *     reference's order (compute.wesl:44-56, rng.wesl:1-21), so a pixel's
+ *     samples run one after another on one lane;
+ *   CG_RNG_SAMPLE (no counterpart in the reference): one stream per sample,
+ *     seeded from the pixel's base seed b (the pixel index on frame 0, the
+ *     carried seed after) as step(step(b) ^ (i + 1) * 0x9E3779B9) for sample
+ *     i, where step is the reference's PCG step; the carried seed is
+ *     step(step(b)).  Samples are traced on separate lanes and summed in
+ *     sample order, so the image does not depend on the schedule.  The same
+ *     estimator with independent streams: images converge to the
+ *     reference-order ones. */
+enum { CG_RNG_REFERENCE = 0, CG_RNG_SAMPLE = 1 };
+int cg_env_set_rng_mode(cg_env* env, int mode);
 int cg_stats_reset(cg_env* env);
 
 /* Upload a flattened scene and build the device layout (quantised 8-wide BVH
@@ -203,7 +218,7 @@ typedef struct cg_job {
   cg_camera camera;
   cg_conf conf;                /* render size, sample_count, trace_depth, lobe caps */
   uint32_t iter_count;         /* 0 starts a new accumulation */
-  uint32_t _pad;
+  uint32_t rng_mode;           /* CG_RNG_REFERENCE or CG_RNG_SAMPLE (cg_view_job; batches: reference) */
   cg_history_pixel* history;   /* device, W*H: read when iter_count > 0, written */
   uint8_t* rgba8;              /* device, W*H*4, nullable */
   uint32_t* path_hash;         /* device, W*H, nullable (path records) */

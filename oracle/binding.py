@@ -51,6 +51,10 @@ def lib() -> C.CDLL:
             ("orc_trace_closest", None, [P, P, U32, P, P, P, P, C.c_int, C.c_int]),
             ("orc_render", C.c_int, [P, C.POINTER(Conf), C.POINTER(Camera), U32, P, P, P, P, U32, U32, C.c_int,
                                      C.c_int]),
+            ("orc_render_mode", C.c_int, [P, C.POINTER(Conf), C.POINTER(Camera), U32, P, P, P, P, U32, U32,
+                                          C.c_int, C.c_int, C.c_int]),
+            ("orc_sample_seed", U32, [U32, U32]),
+            ("orc_sample_carry", U32, [U32]),
             ("orc_sincos", None, [F, FP, FP]),
             ("orc_texture_sample", None, [P, U32, U32, U32, C.c_int32, F, F, FP]),
             ("orc_emissive_texel", None, [C.POINTER(native.TextureC), F, F, FP]),
@@ -198,15 +202,17 @@ class OracleScene:
         return out
 
     def render(self, conf: Conf, camera: Camera, iter_count: int = 0, history: np.ndarray | None = None,
-               rows: tuple[int, int] | None = None, threads: int = 8, cull: bool = True) -> dict:
+               rows: tuple[int, int] | None = None, threads: int = 8, cull: bool = True, rng_mode: int = 0) -> dict:
+        """rng_mode 0: the reference's per-pixel stream; 1: one stream per sample (orc_render_mode)."""
         w, h = conf.render_width, conf.render_height
         hist = np.zeros(w * h, HISTORY) if history is None else np.ascontiguousarray(history.reshape(-1), HISTORY).copy()
         rgba = np.zeros((h, w, 4), np.uint8)
         ph = np.zeros(w * h, np.uint32)
         rc = np.zeros(w * h, np.uint32)
         y0, y1 = rows if rows else (0, h)
-        st = lib().orc_render(self._h, C.byref(conf), C.byref(camera), iter_count, hist.ctypes.data, rgba.ctypes.data,
-                              ph.ctypes.data, rc.ctypes.data, y0, y1, threads, int(cull))
+        st = lib().orc_render_mode(self._h, C.byref(conf), C.byref(camera), iter_count, hist.ctypes.data,
+                                   rgba.ctypes.data, ph.ctypes.data, rc.ctypes.data, y0, y1, threads, int(cull),
+                                   int(rng_mode))
         if st != 0:
             raise RuntimeError("oracle: scene uses features the oracle does not restate")
         return {"history": hist.reshape(h, w), "rgba": rgba, "path_hash": ph.reshape(h, w),

@@ -1588,6 +1588,35 @@ void orc_trace_closest(const orc_scene* s, const cg_ray* rays, uint32_t n, cg_hi
 int orc_render(const orc_scene* s, const cg_conf* conf, const cg_camera* cam, uint32_t iter_count,
                cg_history_pixel* history, uint8_t* rgba8, uint32_t* path_hash, uint32_t* ray_count, uint32_t y0,
                uint32_t y1, int threads, int cull) {
+  return orc_render_mode(s, conf, cam, iter_count, history, rgba8, path_hash, ray_count, y0, y1, threads, cull, 0);
+}
+
+// Sample-stream mode (declared for this build; the reference has one stream
+// per pixel): sample i of a pixel draws from its own stream, seeded from the
+// pixel's base seed b (the pixel index on the first frame, the carried seed
+// later) as step(step(b) ^ (i + 1) * 0x9E3779B9); within the sample the
+// draws follow the reference's depth-first order.  The carried seed is
+// step(step(b)) (the "i = -1" stream, distinct from every sample's because
+// step is a bijection).  The samples' XYZ values are summed in sample order,
+// so the image is the same however the samples are spread over workers; the
+// path hash folds each sample's own hash in sample order.
+uint32_t orc_sample_seed(uint32_t base, uint32_t i) {
+  uint32_t t = base;
+  step_rng(t);
+  t ^= (i + 1u) * 0x9E3779B9u;
+  step_rng(t);
+  return t;
+}
+uint32_t orc_sample_carry(uint32_t base) {
+  uint32_t t = base;
+  step_rng(t);
+  step_rng(t);
+  return t;
+}
+
+int orc_render_mode(const orc_scene* s, const cg_conf* conf, const cg_camera* cam, uint32_t iter_count,
+                    cg_history_pixel* history, uint8_t* rgba8, uint32_t* path_hash, uint32_t* ray_count, uint32_t y0,
+                    uint32_t y1, int threads, int cull, int rng_mode) {
   const uint32_t W = conf->render_width, H = conf->render_height;
   Params P;
   P.D = static_cast<int>(std::min(std::max(conf->trace_depth, 1u), 64u));
@@ -1616,19 +1645,23 @@ int orc_render(const orc_scene* s, const cg_conf* conf, const cg_camera* cam, ui
       // render.wesl:7-18
       v3 acc = mk(0.0f, 0.0f, 0.0f);
       for (uint32_t i = 0; i < conf->sample_count; ++i) {
+        uint32_t sample_hash = 2166136261u;
+        if (rng_mode == 1) rng = orc_sample_seed(seed, i);
+        uint32_t& h = rng_mode == 1 ? sample_hash : hash;
         float lambda, prob;
         sample_wavelength_cdf(rand_float(rng), lambda, prob);
-        Tracer tr{*s, P, lambda, rng, hash, rays, cull != 0, {}};
+        Tracer tr{*s, P, lambda, rng, h, rays, cull != 0, {}};
         float radiance = tr.trace(pos, dir, 0);
         v3 il = illuminate(lambda, radiance);
         float spp = static_cast<float>(conf->sample_count);
         acc = acc + mk(il.x / prob / spp, il.y / prob / spp, il.z / prob / spp);
+        if (rng_mode == 1) hash = hmix(hash, sample_hash);
       }
       lum = lum + acc * w;
       history[hidx].xyz[0] = lum.x;
       history[hidx].xyz[1] = lum.y;
       history[hidx].xyz[2] = lum.z;
-      history[hidx].rand_seed = rand_uint(rng);
+      history[hidx].rand_seed = rng_mode == 1 ? orc_sample_carry(seed) : rand_uint(rng);
       if (rgba8) {
         v3 c = xyz_to_srgb(lum);
         uint8_t* o = rgba8 + static_cast<size_t>(hidx) * 4;

@@ -75,6 +75,18 @@ int orc_render(const orc_scene* s, const cg_conf* conf, const cg_camera* cam,
                uint32_t* path_hash, uint32_t* ray_count,
                uint32_t y0, uint32_t y1, int threads, int cull);
 
+/* orc_render with a choice of RNG mode: 0 the reference's one stream per
+ * pixel (orc_render), 1 one stream per sample -- sample i seeded
+ * orc_sample_seed(b, i) from the pixel's base seed b, XYZ summed in sample
+ * order, carried seed orc_sample_carry(b), path hash the fold of the
+ * samples' hashes in sample order. */
+int orc_render_mode(const orc_scene* s, const cg_conf* conf, const cg_camera* cam,
+                    uint32_t iter_count, cg_history_pixel* history, uint8_t* rgba8,
+                    uint32_t* path_hash, uint32_t* ray_count,
+                    uint32_t y0, uint32_t y1, int threads, int cull, int rng_mode);
+uint32_t orc_sample_seed(uint32_t base, uint32_t i);
+uint32_t orc_sample_carry(uint32_t base);
+
 /* Unit-level pieces, for known-answer and closed-form checks. */
 /* Material texture lookup (bilinear, repeat, level 0) in an array of `layers`
  * w x h RGBA8 images; out receives rgba. */

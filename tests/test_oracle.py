@@ -522,3 +522,63 @@ def test_declared_sincos_against_fp64(oracle):
             ulp = np.spacing(np.float32(max(abs(ref), 2.0 ** -24)))
             worst = max(worst, abs(got - ref) / ulp)
     assert worst <= 2.0
+
+
+# --- Sample-stream mode (declared for this build) ---------------------------
+
+
+def sample_seed_model(base: int, i: int) -> int:
+    t = pcg_hash_model(base)
+    t ^= ((i + 1) * 0x9E3779B9) % 2**32
+    return pcg_hash_model(t)
+
+
+def test_sample_stream_seeds(oracle):
+    """Sample i's stream starts at step(step(b) ^ (i + 1) * 0x9E3779B9) and the
+    carried seed is step(step(b)), against the independent integer model; the
+    seeds of a pixel's samples and its carried seed are pairwise distinct."""
+    L = oracle.lib()
+    rng = np.random.default_rng(23)
+    for base in [0, 1, 2**32 - 1] + [int(x) for x in rng.integers(0, 2**32, 200)]:
+        seeds = [L.orc_sample_seed(base, i) for i in range(64)]
+        assert seeds == [sample_seed_model(base, i) for i in range(64)]
+        carry = L.orc_sample_carry(base)
+        assert carry == pcg_hash_model(pcg_hash_model(base))
+        assert len(set(seeds + [carry])) == 65
+
+
+def test_sample_stream_sky_pixels_closed_form(cg, oracle):
+    """Sky-only pixels in the sample-stream mode: sample i's wavelength comes
+    from the first uniform of its own stream, and the pixel is the in-order
+    sum of 3 xyz(l_i) L(l_i) / sum(xyz(l_i)) / spp; the carried seed is
+    step(step(b))."""
+    s = _empty_scene(cg)
+    orc = oracle.OracleScene.build(s.describe())
+    spp = 5
+    conf = cg.Conf.make(6, 4, spp, 4)
+    cam = cg.Camera.make((0, 0, 0), (1, 0, 0.2), (0.5, 0.5))
+    out = orc.render(conf, cam, rng_mode=1)
+    L = oracle.lib()
+    for py in range(4):
+        for px in range(6):
+            base = px + py * 6
+            ray = np.zeros(1, cg.RAY)
+            L.orc_camera_ray(C.byref(cam), 6, 4, px, py, ray.ctypes.data)
+            d = ray["dir"][0].astype(np.float64)
+            d /= np.linalg.norm(d)
+            vv = math.acos(max(-1, min(1, d[2]))) / math.pi
+            y = min(255, int(min(max(1 - vv, 0), 0.999999) * 256))
+            t = y / 255.0
+            rgb = np.array([0.7, 0.8, 1.0]) * (1 - t) + np.array([0.05, 0.05, 0.06]) * t
+            acc = np.zeros(3, np.float32)
+            for i in range(spp):
+                st = C.c_uint32(sample_seed_model(base, i))
+                u = L.orc_rand_float(C.byref(st))
+                lam, prob = C.c_float(), C.c_float()
+                L.orc_sample_wavelength(u, C.byref(lam), C.byref(prob))
+                Lr = L.orc_rgb_to_radiance((C.c_float * 3)(*rgb), lam.value)
+                xyz = np.array([cmf64(lam.value, LOBES[c]) for c in "xyz"])
+                acc += np.float32(3 * xyz * Lr / xyz.sum() / spp)
+            got = out["history"]["xyz"][py, px]
+            assert np.allclose(got, acc, rtol=5e-5, atol=1e-7), (px, py)
+            assert out["history"]["rand_seed"][py, px] == pcg_hash_model(pcg_hash_model(base))
