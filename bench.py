@@ -265,6 +265,28 @@ def batched_form(cg, seed: int, n_jobs: int, budget_gib: float, log, tune_engine
     vb.release()
     same_mega = sum(h == hashlib.sha1(rj.read_history().tobytes()).hexdigest() for h, rj in zip(ref, rjs))
     log(f"one launch, per-pixel kernel: {t_mega:.2f} s")
+    # The same jobs in the sample-stream RNG mode: one at a time, then in one launch.
+    for rj in rjs:
+        rj.rng_mode = 1
+    env.sync()
+    alone_s, t0 = [], time.perf_counter()
+    for rj in rjs:
+        a = time.perf_counter()
+        rj.view_alone()
+        env.sync()
+        alone_s.append(time.perf_counter() - a)
+    t_alone_s = time.perf_counter() - t0
+    ref_s = [hashlib.sha1(rj.read_history().tobytes()).hexdigest() for rj in rjs]
+    env.sync()
+    t0 = time.perf_counter()
+    vb = cg.ViewBatch(env, [rjs[i] for i in order])
+    vb.wait()
+    t_mega_s = time.perf_counter() - t0
+    vb.release()
+    same_mega_s = sum(h == hashlib.sha1(rj.read_history().tobytes()).hexdigest() for h, rj in zip(ref_s, rjs))
+    for rj in rjs:
+        rj.rng_mode = 0
+    log(f"sample-stream mode: one at a time {t_alone_s:.2f} s, one launch {t_mega_s:.2f} s")
     b = cg.Batch(env, max_trace_depth=depth, **tune.kwargs())
     footprint = b.footprint()[0]
     b.stats(reset=True)
@@ -316,6 +338,14 @@ def batched_form(cg, seed: int, n_jobs: int, budget_gib: float, log, tune_engine
                               "job_latency_ms": {"p50": round(pct(lat, 50) * 1e3, 1),
                                                  "p99": round(pct(lat, 99) * 1e3, 1)},
                               "hbm_bytes": footprint, "identical_jobs": f"{same}/{len(rjs)}"},
+        "batched_sample_stream": {"engine": "one persistent launch of the per-pixel kernels, sample-stream RNG "
+                                            "mode (a pixel's samples on separate waves)",
+                                  "seconds": round(t_mega_s, 3), "Mrays/s": round(rays / t_mega_s / 1e6, 1),
+                                  "identical_jobs": f"{same_mega_s}/{len(rjs)} (vs one at a time in the same mode)",
+                                  "one_at_a_time_seconds": round(t_alone_s, 3),
+                                  "one_at_a_time_Mrays/s": round(rays / t_alone_s / 1e6, 1),
+                                  "speedup_vs_one_at_a_time_same_mode": round(t_alone_s / t_mega_s, 3),
+                                  "speedup_vs_one_at_a_time_reference_order": round(t_alone / t_mega_s, 3)},
         "speedup": round(t_alone / t_mega, 3),
         "speedup_wavefront": round(t_alone / t_batch, 3),
     }
@@ -333,11 +363,16 @@ def serve_run(srv, reqs: list, check: int) -> dict:
     check_set = set(list(range(len(reqs)))[:: max(1, len(reqs) // max(1, check))][:check]) if check else set()
     i, t_start = 0, time.perf_counter()
     t_last = t_start
-    while len(lat) < len(reqs):
+    refused = 0
+    while len(lat) + refused < len(reqs):
         now = time.perf_counter() - t_start
         while i < len(reqs) and reqs[i][0] <= now:
             spec = reqs[i][1]
-            ids[srv.submit(spec.scene, spec.camera(), spec.conf())] = i
+            rid = srv.submit(spec.scene, spec.camera(), spec.conf())
+            if rid is None:
+                refused += 1  # the server's bound on waiting requests turned it away
+            else:
+                ids[rid] = i
             i += 1
         wait = (reqs[i][0] - now) if i < len(reqs) else 0.25
         r = srv.next(max(0.0, min(wait, 0.05)))
@@ -359,6 +394,7 @@ def serve_run(srv, reqs: list, check: int) -> dict:
     lv = list(lat.values())
     out = {
         "requests": len(reqs),
+        "accepted": len(lat), "refused": refused,
         "latency_ms": {"p50": round(pct(lv, 50) * 1e3, 1), "p99": round(pct(lv, 99) * 1e3, 1),
                        "max": round(max(lv) * 1e3, 1)},
         "queued_ms": {"p50": round(pct(list(queued.values()), 50) * 1e3, 1),
@@ -372,7 +408,8 @@ def serve_run(srv, reqs: list, check: int) -> dict:
 
 
 def serving_form(cg, seed: int, rate: float, seconds: float, n_scenes: int, scene_scale: float, budget_gib: float,
-                 check: int, log, texture_size: int = 2048, sustainable: float = 0.8) -> dict:
+                 check: int, log, texture_size: int = 2048, sustainable: float = 0.8, rng_mode: int = 0,
+                 overload: tuple | None = None) -> dict:
     """The serving form: n_scenes resident scenes (16-sphere generator at
     `scene_scale`, each with four texture_size^2 RGBA8 material textures:
     base colour, metallic-roughness, transmission, emission), Poisson
@@ -386,21 +423,28 @@ def serving_form(cg, seed: int, rate: float, seconds: float, n_scenes: int, scen
     times the capacity reports the latency of a load the GPU keeps up with.
     The first 48 requests rendered one at a time with the plain kernel give
     the one-at-a-time rate of the same mix, which the served rate is
-    compared with."""
+    compared with.  rng_mode 1 serves every request (and renders the
+    one-at-a-time ones) in the sample-stream RNG mode.  `overload` = (rate,
+    seconds, max_waiting): afterwards a burst above capacity against a server
+    that refuses requests once max_waiting wait, reporting accepted and
+    refused requests and the accepted ones' latency."""
     from crystalgraphics_amd import workloads
     t0 = time.perf_counter()
     params = [scene_scale, 1, texture_size] if texture_size > 0 else [scene_scale]
     hosts = [cg.HostScene.generate("c1", seed=21 + i, params=params) for i in range(n_scenes)]
     tris = workloads.scene_triangles(scene_scale) * n_scenes
-    srv = cg.Server(hosts, budget_bytes=int(budget_gib * (1 << 30)), max_trace_depth=8)
+    srv = cg.Server(hosts, budget_bytes=int(budget_gib * (1 << 30)), max_trace_depth=8, rng_mode=rng_mode)
     setup = time.perf_counter() - t0
     reqs = workloads.serve_requests(rate, seconds, n_scenes, seed=seed)
-    log(f"serving: {len(reqs)} requests over {seconds:.0f} s at {rate:.0f}/s, setup {setup:.1f} s")
+    log(f"serving (rng mode {rng_mode}): {len(reqs)} requests over {seconds:.0f} s at {rate:.0f}/s, "
+        f"setup {setup:.1f} s")
     out = {
         "workload": f"{n_scenes} resident scenes, {tris / 1e6:.1f}M tris total, "
                     f"{4 if texture_size > 0 else 0} x {texture_size}^2 RGBA8 textures per scene; Poisson {rate:.0f} req/s for "
                     f"{seconds:.0f} s; 256x256/640x360/1280x720/1920x1080 weighted 4:3:2:1, spp {{1,4,16}}, depth 8",
         "setup_s": round(setup, 1), "budget_gib": budget_gib, "rate_req_per_s": rate,
+        "rng": "reference order (one stream per pixel)" if rng_mode == 0 else
+               "sample-stream mode (one stream per sample; results equal the alone render in the same mode)",
     }
     out.update(serve_run(srv, reqs, check))
     per_req = out["rays"] / max(1, out["requests"])
@@ -419,7 +463,7 @@ def serving_form(cg, seed: int, rate: float, seconds: float, n_scenes: int, scen
     out["queries_per_request"] = round(per_req / 1e6, 2)
     out["offered_Mrays_per_s"] = round(rate * per_req / 1e6, 1)
     out["capacity_req_per_s"] = round(cap, 2)
-    if sustainable > 0:
+    if sustainable > 0 and cap < rate:  # above capacity latency measures the run's length: also report a load it keeps up with
         r2 = sustainable * cap
         reqs2 = workloads.serve_requests(r2, seconds, n_scenes, seed=seed + 1)
         log(f"serving: {len(reqs2)} requests at {r2:.1f}/s ({sustainable:.0%} of capacity)")
@@ -427,6 +471,19 @@ def serving_form(cg, seed: int, rate: float, seconds: float, n_scenes: int, scen
         sub["rate_req_per_s"] = round(r2, 2)
         out["at_sustainable_rate"] = sub
     srv.close()
+    if overload:
+        o_rate, o_seconds, o_wait = overload
+        srv = cg.Server(hosts, budget_bytes=int(budget_gib * (1 << 30)), max_trace_depth=8, rng_mode=rng_mode,
+                        max_waiting=o_wait)
+        reqs3 = workloads.serve_requests(o_rate, o_seconds, n_scenes, seed=seed + 2)
+        log(f"serving overload: {len(reqs3)} requests over {o_seconds:.0f} s at {o_rate:.0f}/s, "
+            f"at most {o_wait} waiting")
+        sub = serve_run(srv, reqs3, 0)
+        sub["rate_req_per_s"] = o_rate
+        sub["max_waiting"] = o_wait
+        sub["accepted_req_per_s"] = round(sub["accepted"] / max(sub["span_s"], 1e-9), 2)
+        out["overload"] = sub
+        srv.close()
     return out
 
 
@@ -443,7 +500,14 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-form section")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-form section")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="requests per second (Poisson)")
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
+                    help="length of the arrival stream served in the sample-stream RNG mode")
+    ap.add_argument("--serve-ref-seconds", type=float, default=10.0,
+                    help="length of the arrival stream served in reference order")
+    ap.add_argument("--overload-rate", type=float, default=200.0, help="overload burst: requests per second")
+    ap.add_argument("--overload-seconds", type=float, default=5.0, help="overload burst length (0: skip)")
+    ap.add_argument("--overload-waiting", type=int, default=32,
+                    help="overload burst: requests allowed to wait before the server refuses")
     ap.add_argument("--serve-scenes", type=int, default=8)
     ap.add_argument("--serve-scene-scale", type=float, default=1.5390625,
                     help="sphere subdivision factor of each scene (1.539: ~4.9M tris per scene)")
@@ -600,9 +664,14 @@ def main() -> int:
 
     serving = None
     if not args.no_serve:
-        serving = serving_form(cg, args.seed, args.serve_rate, args.serve_seconds, args.serve_scenes,
-                               args.serve_scene_scale, args.serve_budget_gib, 4,
-                               lambda m: print(m, file=sys.stderr, flush=True), args.serve_texture_size)
+        log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+        serving = serving_form(cg, args.seed, args.serve_rate, args.serve_ref_seconds, args.serve_scenes,
+                               args.serve_scene_scale, args.serve_budget_gib, 4, log, args.serve_texture_size)
+        serving["sample_stream"] = serving_form(
+            cg, args.seed, args.serve_rate, args.serve_seconds, args.serve_scenes, args.serve_scene_scale,
+            args.serve_budget_gib, 4, log, args.serve_texture_size, rng_mode=1,
+            overload=(args.overload_rate, args.overload_seconds, args.overload_waiting)
+            if args.overload_seconds > 0 else None)
 
     base = None
     if not args.no_cpu:
@@ -654,7 +723,7 @@ def main() -> int:
         "traversal_microbench": micro,
         "batched_form": batched,
         "serving_form": serving,
-        "latency_ms": None if serving is None else serving["latency_ms"],
+        "latency_ms": None if serving is None else serving["sample_stream"]["latency_ms"],
     }
     print(json.dumps(out))
     env.close()

@@ -522,12 +522,13 @@ class Server:
 
     def __init__(self, scenes: list[HostScene], budget_bytes: int = 16 << 30, engine: int = 0,
                  output_fraction: float = 0.25, max_trace_depth: int = 8, in_flight: int = 4,
-                 rounds_per_launch: int = 16, block_size: int = 0, short_stack: int = 0, max_waiting: int = 0):
+                 rounds_per_launch: int = 16, block_size: int = 0, short_stack: int = 0, max_waiting: int = 0,
+                 rng_mode: int = 0):
         arr = (C.c_void_p * len(scenes))(*[s.handle for s in scenes])
         h = C.c_void_p()
         rc = native.host().cgh_server_create(arr, len(scenes), engine, int(budget_bytes), float(output_fraction),
                                              max_trace_depth, in_flight, rounds_per_launch, block_size, short_stack,
-                                             max_waiting, C.byref(h))
+                                             max_waiting, rng_mode, C.byref(h))
         if rc != 0:
             raise NativeError(rc, native.host().cgh_serve_last_error().decode())
         self._h = h.value

@@ -62,8 +62,11 @@ struct cg_vbatch {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
   DevBuf jobs, first, queue, ovf;
+  DevBuf jobs_s, first_s, pfirst_s, samples;  // the launch's sample-stream jobs
   std::vector<ViewArgs> host_jobs;  // sources of the in-flight copies: kept until reuse
   std::vector<uint32_t> host_first;
+  std::vector<ViewArgs> host_jobs_s;
+  std::vector<uint32_t> host_first_s, host_pfirst_s;
   hipGraphExec_t exec = nullptr;  // the last launch's graph (counter reset + kernel)
   bool in_flight = false;
 };
@@ -448,6 +451,57 @@ __global__ __launch_bounds__(kBlock, 4) void k_view_batch(const ViewArgs* __rest
     if (px < A.W && py < A.H) render_pixel<MF>(A, px, py, stk, st, nrays, nshadow);
   }
   if (stats) flush_stats(stats, nrays, nshadow, st.nodes, st.tris, st.insts);
+}
+
+// The job of work item `item`: first[lo] <= item < first[lo + 1].
+__device__ __forceinline__ uint32_t job_of_item(const uint32_t* __restrict__ first, uint32_t njobs, uint32_t item) {
+  uint32_t lo = 0, hi = njobs;
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (first[mid] <= item)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+// Sample-stream jobs of a many-job launch: the items are every job's (tile,
+// sample chunk) pairs; k_view_batch_finish then sums each pixel's samples.
+template <int MF>
+__global__ __launch_bounds__(kBlock, 4) void k_view_batch_s(const ViewArgs* __restrict__ jobs,
+                                                            const uint32_t* __restrict__ first, uint32_t njobs,
+                                                            WorkQueue q, StackArgs sa,
+                                                            unsigned long long* __restrict__ stats) {
+  __shared__ uint32_t s_ref[kShortStack * kBlock];
+  __shared__ float s_t[kShortStack * kBlock];
+  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kShortStack);
+  TravStats st{0u, 0u, 0u};
+  uint32_t nrays = 0, nshadow = 0;
+  uint32_t probe = 0;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (;;) {
+    const uint32_t item = grab_item(q, probe);
+    if (item == kNoItem) break;
+    const uint32_t j = job_of_item(first, njobs, item);
+    const ViewArgs& A = jobs[j];
+    const uint32_t local = item - first[j];
+    const uint32_t tile = local / A.nchunks, c = local - tile * A.nchunks;
+    const uint32_t tiles_x = (A.W + 7u) / 8u;
+    const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u);
+    const uint32_t py = (tile / tiles_x) * 8u + (lane >> 3);
+    const uint32_t s0 = c * A.chunk, s1 = s0 + A.chunk < A.spp ? s0 + A.chunk : A.spp;
+    if (px < A.W && py < A.H) render_samples<MF>(A, SampleBufs{A.sxyz, A.srec}, px, py, s0, s1, stk, st, nrays, nshadow);
+  }
+  if (stats) flush_stats(stats, nrays, nshadow, st.nodes, st.tris, st.insts);
+}
+__global__ __launch_bounds__(kBlock) void k_view_batch_finish(const ViewArgs* __restrict__ jobs,
+                                                              const uint32_t* __restrict__ pfirst, uint32_t njobs,
+                                                              uint32_t npix) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= npix) return;
+  const uint32_t j = job_of_item(pfirst, njobs, i);
+  const ViewArgs& A = jobs[j];
+  finish_pixel(A, SampleBufs{A.sxyz, A.srec}, i - pfirst[j]);
 }
 
 // Linear-BVH build of one BLAS (cg_lbvh.h): one thread per triangle, per
@@ -1159,7 +1213,8 @@ int cg_env_destroy(cg_env* env) {
   dev_free(env->samples);
   for (cg_vbatch* v : env->vbatch_all) {
     if (v->stream) (void)hipStreamSynchronize(v->stream);
-    for (DevBuf* d : {&v->jobs, &v->first, &v->queue, &v->ovf}) dev_free(*d);
+    for (DevBuf* d : {&v->jobs, &v->first, &v->queue, &v->ovf, &v->jobs_s, &v->first_s, &v->pfirst_s, &v->samples})
+      dev_free(*d);
     if (v->exec) (void)hipGraphExecDestroy(v->exec);
     if (v->done) (void)hipEventDestroy(v->done);
     if (v->stream) (void)hipStreamDestroy(v->stream);
@@ -1591,6 +1646,17 @@ template <int MF>
 uint32_t view_batch_grid(cg_env* env, uint32_t items) {
   return resident_grid(k_view_batch<MF>, env, (items + 3u) / 4u);
 }
+template <int MF>
+void launch_view_batch_s_mf(cg_vbatch* v, uint32_t grid, uint32_t njobs, const WorkQueue& q, const StackArgs& sa,
+                            unsigned long long* stats) {
+  hipLaunchKernelGGL((k_view_batch_s<MF>), dim3(grid), dim3(kBlock), 0, v->stream,
+                     static_cast<const ViewArgs*>(v->jobs_s.p), static_cast<const uint32_t*>(v->first_s.p), njobs, q,
+                     sa, stats);
+}
+template <int MF>
+uint32_t view_batch_s_grid(cg_env* env, uint32_t items) {
+  return resident_grid(k_view_batch_s<MF>, env, (items + 3u) / 4u);
+}
 }  // namespace
 
 extern "C" {
@@ -1599,10 +1665,7 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
   if (!env || !out || (n && !jobs)) return fail(env, CG_ERR_INVALID, "cg_view_batch_launch: null argument");
   *out = nullptr;
   if (n == 0) return fail(env, CG_ERR_INVALID, "cg_view_batch_launch: no jobs");
-  std::vector<ViewArgs> va(n);
-  std::vector<uint32_t> first(n + 1, 0u);
-  uint32_t depth = 1, frames = 1;
-  // Tiles in order of their expected cost (samples x trace depth per pixel),
+  // Jobs in order of their expected cost (samples x trace depth per pixel),
   // dealt round-robin to the XCDs: every XCD starts on its most expensive
   // tiles and the launch ends on cheap ones instead of a few long tiles.
   std::vector<uint32_t> ord(n);
@@ -1613,6 +1676,12 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
   };
   std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
 #endif
+  std::vector<ViewArgs> va, vs;                      // reference-order jobs, sample-stream jobs
+  std::vector<uint32_t> first{0u}, first_s{0u}, pfirst_s{0u};
+  std::vector<size_t> sample_off;                     // each sample-stream job's first sample slot
+  size_t samples = 0;
+  bool records_s = false;
+  uint32_t depth = 1, frames = 1, frames_s = 1;
   for (uint32_t k = 0; k < n; ++k) {
     const uint32_t i = ord[k];
     const cg_job* job = jobs + i;
@@ -1622,8 +1691,9 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
     const cg_conf& c = job->conf;
     if (c.render_width == 0 || c.render_height == 0) return fail(env, CG_ERR_INVALID, "job %u: empty render size", i);
     if (job->path_hash && !job->ray_count) return fail(env, CG_ERR_INVALID, "job %u: path_hash needs ray_count", i);
-    ViewArgs& a = va[k];
-    a = ViewArgs{};
+    if (job->rng_mode != CG_RNG_REFERENCE && job->rng_mode != CG_RNG_SAMPLE)
+      return fail(env, CG_ERR_INVALID, "job %u: unknown rng_mode %u", i, job->rng_mode);
+    ViewArgs a{};
     a.S = job->scene->view;
     a.P.D = static_cast<int>(std::min(std::max(c.trace_depth, 1u), 64u));
     a.P.T = c.max_transport_sample_count;
@@ -1634,9 +1704,9 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
     a.H = c.render_height;
     a.spp = c.sample_count;
     a.iter = job->iter_count;
-    for (int k = 0; k < 3; ++k) {
-      a.cam_pos[k] = job->camera.position[k];
-      a.cam_dir[k] = job->camera.direction[k];
+    for (int d = 0; d < 3; ++d) {
+      a.cam_pos[d] = job->camera.position[d];
+      a.cam_dir[d] = job->camera.direction[d];
     }
     a.viewport[0] = job->camera.viewport[0];
     a.viewport[1] = job->camera.viewport[1];
@@ -1649,10 +1719,28 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
     a.ray_count = job->path_hash ? job->ray_count : nullptr;
     a.stats = nullptr;
     const uint64_t tiles = static_cast<uint64_t>((c.render_width + 7u) / 8u) * ((c.render_height + 7u) / 8u);
-    if (first[k] + tiles > 0x7FFFFF00ull) return fail(env, CG_ERR_INVALID, "batch too large");
-    first[k + 1] = first[k] + static_cast<uint32_t>(tiles);
     depth = std::max(depth, job->scene->stack_depth);
-    frames = std::max(frames, static_cast<uint32_t>(std::max(1, a.P.D - 1)));
+    const uint32_t fr = static_cast<uint32_t>(std::max(1, a.P.D - 1));
+    if (job->rng_mode == CG_RNG_SAMPLE && a.spp > 0u) {
+      a.chunk = sample_chunk(a.spp);
+      a.nchunks = (a.spp + a.chunk - 1u) / a.chunk;
+      const uint64_t items = tiles * a.nchunks;
+      const uint64_t px = static_cast<uint64_t>(a.W) * a.H;
+      if (first_s.back() + items > 0x7FFFFF00ull || pfirst_s.back() + px > 0x7FFFFF00ull)
+        return fail(env, CG_ERR_INVALID, "batch too large");
+      first_s.push_back(first_s.back() + static_cast<uint32_t>(items));
+      pfirst_s.push_back(pfirst_s.back() + static_cast<uint32_t>(px));
+      sample_off.push_back(samples);
+      samples += px * a.spp;
+      records_s = records_s || a.path_hash;
+      frames_s = std::max(frames_s, fr);
+      vs.push_back(a);
+    } else {
+      if (first.back() + tiles > 0x7FFFFF00ull) return fail(env, CG_ERR_INVALID, "batch too large");
+      first.push_back(first.back() + static_cast<uint32_t>(tiles));
+      frames = std::max(frames, fr);
+      va.push_back(a);
+    }
   }
   cg_vbatch* v = nullptr;
   if (!env->vbatch_pool.empty()) {
@@ -1672,32 +1760,63 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
     env->vbatch_pool.push_back(v);
     return rc;
   };
-  const uint32_t items = first[n];
-  const uint32_t grid = frames <= 4    ? view_batch_grid<4>(env, items)
+  const uint32_t nref = static_cast<uint32_t>(va.size()), ns = static_cast<uint32_t>(vs.size());
+  const uint32_t items = first.back(), items_s = first_s.back(), npix_s = pfirst_s.back();
+  const uint32_t grid = nref == 0 ? 0u
+                        : frames <= 4  ? view_batch_grid<4>(env, items)
                         : frames <= 8  ? view_batch_grid<8>(env, items)
                         : frames <= 16 ? view_batch_grid<16>(env, items)
                                        : view_batch_grid<64>(env, items);
+  const uint32_t grid_s = ns == 0 ? 0u
+                          : frames_s <= 4  ? view_batch_s_grid<4>(env, items_s)
+                          : frames_s <= 8  ? view_batch_s_grid<8>(env, items_s)
+                          : frames_s <= 16 ? view_batch_s_grid<16>(env, items_s)
+                                           : view_batch_s_grid<64>(env, items_s);
   int rc = CG_OK;
-  if ((rc = grow(env, v->jobs, n * sizeof(ViewArgs))) != CG_OK) return give_back(rc);
-  if ((rc = grow(env, v->first, (n + 1) * sizeof(uint32_t))) != CG_OK) return give_back(rc);
   if ((rc = grow(env, v->queue, kXcds * kCounterStride * sizeof(uint32_t))) != CG_OK) return give_back(rc);
+  if (nref) {
+    if ((rc = grow(env, v->jobs, nref * sizeof(ViewArgs))) != CG_OK) return give_back(rc);
+    if ((rc = grow(env, v->first, (nref + 1) * sizeof(uint32_t))) != CG_OK) return give_back(rc);
+  }
+  if (ns) {
+    if ((rc = grow(env, v->jobs_s, ns * sizeof(ViewArgs))) != CG_OK) return give_back(rc);
+    if ((rc = grow(env, v->first_s, (ns + 1) * sizeof(uint32_t))) != CG_OK) return give_back(rc);
+    if ((rc = grow(env, v->pfirst_s, (ns + 1) * sizeof(uint32_t))) != CG_OK) return give_back(rc);
+    const size_t bytes = samples * (sizeof(float4) + (records_s ? sizeof(uint2) : 0));
+    if ((rc = grow(env, v->samples, bytes)) != CG_OK) return give_back(rc);
+    float4* xyz = static_cast<float4*>(v->samples.p);
+    uint2* rec = records_s ? reinterpret_cast<uint2*>(xyz + samples) : nullptr;
+    for (uint32_t k = 0; k < ns; ++k) {
+      vs[k].sxyz = xyz + sample_off[k];
+      vs[k].srec = vs[k].path_hash ? rec + sample_off[k] : nullptr;
+    }
+  }
   StackArgs sa{nullptr, 0, static_cast<uint32_t*>(env->errflag.p)};
   if (depth > static_cast<uint32_t>(kShortStack)) {
     sa.ovf_cap = static_cast<int>(depth) - kShortStack;
-    if ((rc = grow(env, v->ovf, static_cast<size_t>(grid) * kBlock * sa.ovf_cap * sizeof(uint2))) != CG_OK)
+    if ((rc = grow(env, v->ovf, static_cast<size_t>(std::max(grid, grid_s)) * kBlock * sa.ovf_cap * sizeof(uint2))) !=
+        CG_OK)
       return give_back(rc);
     sa.ovf = static_cast<uint2*>(v->ovf.p);
   }
   v->host_jobs = std::move(va);
   v->host_first = std::move(first);
-  if (hipMemcpyAsync(v->jobs.p, v->host_jobs.data(), n * sizeof(ViewArgs), hipMemcpyHostToDevice, v->stream) !=
-          hipSuccess ||
-      hipMemcpyAsync(v->first.p, v->host_first.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
-                     v->stream) != hipSuccess)
+  v->host_jobs_s = std::move(vs);
+  v->host_first_s = std::move(first_s);
+  v->host_pfirst_s = std::move(pfirst_s);
+  auto h2d = [&](DevBuf& d, const void* src, size_t bytes) {
+    return bytes == 0 || hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, v->stream) == hipSuccess;
+  };
+  if (!h2d(v->jobs, v->host_jobs.data(), nref * sizeof(ViewArgs)) ||
+      !h2d(v->first, v->host_first.data(), nref ? (nref + 1) * sizeof(uint32_t) : 0) ||
+      !h2d(v->jobs_s, v->host_jobs_s.data(), ns * sizeof(ViewArgs)) ||
+      !h2d(v->first_s, v->host_first_s.data(), ns ? (ns + 1) * sizeof(uint32_t) : 0) ||
+      !h2d(v->pfirst_s, v->host_pfirst_s.data(), ns ? (ns + 1) * sizeof(uint32_t) : 0))
     return give_back(fail(env, CG_ERR_HIP, "cg_view_batch_launch: job table copy failed"));
-  // The counter reset (a kernel, not a memset) and the persistent kernel are
-  // captured as one hipGraph and launched as such.  The vbatch came from the
-  // pool, so its previous graph has finished.
+  // The counter resets (kernels, not memsets) and the persistent kernels --
+  // reference-order jobs, then sample-stream jobs and their per-pixel sums --
+  // are captured as one hipGraph and launched as such.  The vbatch came from
+  // the pool, so its previous graph has finished.
   if (v->exec) {
     (void)hipGraphExecDestroy(v->exec);
     v->exec = nullptr;
@@ -1706,17 +1825,34 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
   if (hipStreamBeginCapture(v->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return give_back(fail(env, CG_ERR_HIP, "cg_view_batch_launch: graph capture failed to begin"));
   uint32_t* ctr = static_cast<uint32_t*>(v->queue.p);
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kXcds * kCounterStride), 0, v->stream, ctr, kXcds * kCounterStride);
-  const WorkQueue q{ctr, items, CG_VB_LPT ? 1u : 0u};
   unsigned long long* stats = env->counting ? static_cast<unsigned long long*>(env->stats.p) : nullptr;
-  if (frames <= 4)
-    launch_view_batch_mf<4>(v, grid, n, q, sa, stats);
-  else if (frames <= 8)
-    launch_view_batch_mf<8>(v, grid, n, q, sa, stats);
-  else if (frames <= 16)
-    launch_view_batch_mf<16>(v, grid, n, q, sa, stats);
-  else
-    launch_view_batch_mf<64>(v, grid, n, q, sa, stats);
+  if (nref) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kXcds * kCounterStride), 0, v->stream, ctr, kXcds * kCounterStride);
+    const WorkQueue q{ctr, items, CG_VB_LPT ? 1u : 0u};
+    if (frames <= 4)
+      launch_view_batch_mf<4>(v, grid, nref, q, sa, stats);
+    else if (frames <= 8)
+      launch_view_batch_mf<8>(v, grid, nref, q, sa, stats);
+    else if (frames <= 16)
+      launch_view_batch_mf<16>(v, grid, nref, q, sa, stats);
+    else
+      launch_view_batch_mf<64>(v, grid, nref, q, sa, stats);
+  }
+  if (ns) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kXcds * kCounterStride), 0, v->stream, ctr, kXcds * kCounterStride);
+    const WorkQueue q{ctr, items_s, CG_VB_LPT ? 1u : 0u};
+    if (frames_s <= 4)
+      launch_view_batch_s_mf<4>(v, grid_s, ns, q, sa, stats);
+    else if (frames_s <= 8)
+      launch_view_batch_s_mf<8>(v, grid_s, ns, q, sa, stats);
+    else if (frames_s <= 16)
+      launch_view_batch_s_mf<16>(v, grid_s, ns, q, sa, stats);
+    else
+      launch_view_batch_s_mf<64>(v, grid_s, ns, q, sa, stats);
+    hipLaunchKernelGGL(k_view_batch_finish, dim3((npix_s + kBlock - 1u) / kBlock), dim3(kBlock), 0, v->stream,
+                       static_cast<const ViewArgs*>(v->jobs_s.p), static_cast<const uint32_t*>(v->pfirst_s.p), ns,
+                       npix_s);
+  }
   const hipError_t le = hipGetLastError();
   hipGraph_t graph = nullptr;
   const hipError_t ce = hipStreamEndCapture(v->stream, &graph);
@@ -2121,6 +2257,8 @@ int cg_batch_submit(cg_batch* b, const cg_job* job, uint32_t* job_id) {
   cg_env* env = b->env;
   const cg_conf& c = job->conf;
   if (c.render_width == 0 || c.render_height == 0) return fail(env, CG_ERR_INVALID, "job render size must be positive");
+  if (job->rng_mode != CG_RNG_REFERENCE)
+    return fail(env, CG_ERR_UNSUPPORTED, "cg_batch_submit: the wavefront engine renders in reference order only");
   const uint32_t depth = std::min(std::max(c.trace_depth, 1u), 64u);
   if (depth > b->conf.max_trace_depth)
     return fail(env, CG_ERR_INVALID, "job trace_depth %u exceeds the batch's %u", depth, b->conf.max_trace_depth);

@@ -25,6 +25,11 @@ struct ViewArgs {
   uint32_t* path_hash;
   uint32_t* ray_count;
   unsigned long long* stats;
+  // sample-stream mode in a many-job launch: the job's per-sample buffers and
+  // its split of the samples into work items
+  float4* sxyz;
+  uint2* srec;
+  uint32_t chunk, nchunks;
 };
 
 

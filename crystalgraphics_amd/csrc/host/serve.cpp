@@ -198,6 +198,7 @@ struct Server::Impl {
     j.camera = to_cg_camera(r.req.camera);
     j.conf = to_cg_conf(r.req.conf);
     j.iter_count = 0;
+    j.rng_mode = conf.rng_mode;
     j.history = reinterpret_cast<cg_history_pixel*>(r.out);
     return j;
   }
@@ -438,6 +439,9 @@ expected<std::unique_ptr<Server>, Error> Server::Create(const ServeConf& conf, c
     return unexpected<Error>(Error{"Server: max_trace_depth must be in [1, 64]"});
   if (!(conf.output_fraction > 0.0 && conf.output_fraction < 1.0))
     return unexpected<Error>(Error{"Server: output_fraction must be in (0, 1)"});
+  if (conf.rng_mode > 1) return unexpected<Error>(Error{"Server: rng_mode must be 0 or 1"});
+  if (conf.rng_mode == 1 && conf.engine == ServeConf::Engine::kWavefront)
+    return unexpected<Error>(Error{"Server: the wavefront engine renders in reference order only (rng_mode 0)"});
   auto impl = std::make_unique<Impl>();
   impl->conf = conf;
   cg_conf c{8, 8, 8, 8, 1, conf.max_trace_depth, 3, 1, 1, 1};
@@ -585,7 +589,8 @@ const char* cgh_serve_last_error(void) { return t_serve_err.c_str(); }
 
 int cgh_server_create(const cgh_scene* const* scenes, uint32_t n, uint32_t engine, uint64_t budget_bytes,
                       double output_fraction, uint32_t max_trace_depth, uint32_t in_flight, uint32_t rounds_per_launch,
-                      uint32_t block_size, uint32_t short_stack, uint32_t max_waiting, cgh_server** out) {
+                      uint32_t block_size, uint32_t short_stack, uint32_t max_waiting, uint32_t rng_mode,
+                      cgh_server** out) {
   if (!out || (n && !scenes)) return serve_fail("cgh_server_create: null argument");
   *out = nullptr;
   std::vector<const Scene*> sc;
@@ -600,6 +605,7 @@ int cgh_server_create(const cgh_scene* const* scenes, uint32_t n, uint32_t engin
   c.block_size = block_size;
   c.short_stack = short_stack;
   c.max_waiting = max_waiting;
+  c.rng_mode = rng_mode;
   auto s = Server::Create(c, sc);
   if (!s) return serve_fail(s.error());
   *out = reinterpret_cast<cgh_server*>(std::move(*s).release());

@@ -308,7 +308,7 @@ def host() -> C.CDLL:
         _sig(h, "cgh_load_scene", I32, C.c_char_p, C.POINTER(P))
         _sig(h, "cgh_serve_last_error", C.c_char_p)
         _sig(h, "cgh_server_create", I32, C.POINTER(P), U32, U32, C.c_uint64, C.c_double, U32, U32, U32, U32, U32,
-             U32, C.POINTER(P))
+             U32, U32, C.POINTER(P))
         _sig(h, "cgh_server_free", None, P)
         _sig(h, "cgh_server_submit", I32, P, U32, C.POINTER(Camera), C.POINTER(Conf), C.POINTER(C.c_uint64))
         _sig(h, "cgh_server_next", I32, P, C.c_double, C.POINTER(ResultInfo), P, C.c_size_t)

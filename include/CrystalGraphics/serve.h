@@ -37,6 +37,10 @@ struct ServeConf {
   // Overload policy: Submit refuses a request (error "Submit: queue full")
   // while this many requests wait for admission; 0 leaves the queue unbounded.
   std::uint32_t max_waiting = 0;
+  // Random-number streams of every render (cg_device.h): 0 the reference's
+  // one stream per pixel, 1 one stream per sample (kLaunches only: a pixel's
+  // samples on separate waves).  RenderAlone uses the same mode.
+  std::uint32_t rng_mode = 0;
 };
 
 struct RenderRequest {

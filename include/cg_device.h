@@ -218,7 +218,8 @@ typedef struct cg_job {
   cg_camera camera;
   cg_conf conf;                /* render size, sample_count, trace_depth, lobe caps */
   uint32_t iter_count;         /* 0 starts a new accumulation */
-  uint32_t rng_mode;           /* CG_RNG_REFERENCE or CG_RNG_SAMPLE (cg_view_job; batches: reference) */
+  uint32_t rng_mode;           /* CG_RNG_REFERENCE or CG_RNG_SAMPLE (cg_view_job, cg_view_batch_launch;
+                                  the wavefront engine takes CG_RNG_REFERENCE only) */
   cg_history_pixel* history;   /* device, W*H: read when iter_count > 0, written */
   uint8_t* rgba8;              /* device, W*H*4, nullable */
   uint32_t* path_hash;         /* device, W*H, nullable (path records) */

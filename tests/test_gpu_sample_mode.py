@@ -95,3 +95,48 @@ def test_sample_mode_converges_to_reference_order(cg, name, params, w, h, depth)
     print(f"{name}: relative MSE sample vs reference order: 1024 spp {rmse[1024]:.2e}, 4096 spp {rmse[4096]:.2e}")
     assert rmse[4096] < 1e-3
     assert rmse[4096] < rmse[1024]
+
+
+def test_view_batch_mixed_modes_equal_alone(cg):
+    """One persistent launch holding reference-order and sample-stream jobs:
+    every job equals the same job rendered alone in its mode, bit for bit
+    (history, rgba8, path records)."""
+    scenes = [cg.HostScene.generate("spheres", seed=3, params=[8]), cg.HostScene.generate("c1", seed=2, params=[0.125])]
+    specs = [(0, 40, 24, 3, 4, 0), (1, 72, 40, 2, 8, 1), (0, 17, 9, 40, 3, 1), (1, 24, 24, 1, 5, 0),
+             (1, 33, 20, 20, 3, 1), (0, 48, 32, 2, 4, 1)]
+    with cg.DeviceEnv(cg.Conf.make(8, 8)) as env:
+        ds = [env.upload(s.scene_data()) for s in scenes]
+        cams = [s.describe()["camera"] for s in scenes]
+        jobs = [cg.RenderJob(env, ds[sc], cams[sc], cg.Conf.make(w, h, spp, d), path_records=True, rng_mode=m)
+                for sc, w, h, spp, d, m in specs]
+        for j in jobs:
+            j.view_alone()
+        ref = [(j.read_history().tobytes(), j.read_rgba8().tobytes(), [a.tobytes() for a in j.read_path_records()])
+               for j in jobs]
+        vb = cg.ViewBatch(env, jobs)
+        vb.wait()
+        vb.release()
+        for k, j in enumerate(jobs):
+            got = (j.read_history().tobytes(), j.read_rgba8().tobytes(), [a.tobytes() for a in j.read_path_records()])
+            assert got == ref[k], f"job {k}"
+        for j in jobs:
+            j.free()
+
+
+def test_serve_in_sample_mode_equals_alone(cg):
+    scenes = [cg.HostScene.generate("spheres", seed=4, params=[6]),
+              cg.HostScene.generate("c1", seed=3, params=[0.125, 1, 256])]
+    cams = [s.describe()["camera"] for s in scenes]
+    reqs = [(k % 2, cg.Conf.make(*[(32, 24), (48, 40), (17, 11), (64, 36)][k % 4], [1, 4, 16][k % 3], [2, 5, 8][k % 3]))
+            for k in range(9)]
+    with cg.Server(scenes, budget_bytes=256 << 20, max_trace_depth=8, rng_mode=1) as srv:
+        ids = {srv.submit(s, cams[s], conf): i for i, (s, conf) in enumerate(reqs)}
+        got = {}
+        while len(got) < len(reqs):
+            r = srv.next(5.0)
+            assert r is not None
+            got[ids[r["id"]]] = r
+        for i, (s, conf) in enumerate(reqs):
+            assert got[i]["history"].tobytes() == srv.render_alone(s, cams[s], conf)["history"].tobytes()
+    with pytest.raises(cg.NativeError, match="reference order"):
+        cg.Server(scenes, budget_bytes=256 << 20, engine=1, rng_mode=1)
