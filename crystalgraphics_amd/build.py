@@ -45,7 +45,7 @@ def build_host() -> Path:
     LIB.mkdir(exist_ok=True)
     out = LIB / "libcrystal_graphics.so"
     srcs = [str(PKG / "csrc/host" / f) for f in ("scene_prep.cpp", "scene_gen.cpp", "graphics.cpp", "serve.cpp",
-                                                              "gltf.cpp")]
+                                                              "gltf.cpp", "jpeg.cpp")]
     _run(["g++", *HOST_FLAGS, "-shared", "-o", str(out), *srcs,
           f"-L{LIB}", "-lcg_device", "-Wl,-rpath,$ORIGIN", "-lpthread", "-lz"])
     return out

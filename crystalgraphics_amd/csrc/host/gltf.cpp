@@ -15,14 +15,15 @@
 //     (matrix, or T * R * S), all under a +90 degree rotation about x that
 //     turns glTF's y-up into z-up (:969-1000);
 //   * textures: images decoded to RGBA8 (PNG here: zlib inflate and the five
-//     PNG filters; 16-bit channels keep their high byte), samplers' wrap
-//     modes kept for the emissive buffer;
+//     PNG filters; 16-bit channels keep their high byte; JPEG: jpeg.cpp,
+//     sequential and progressive), samplers' wrap modes kept for the
+//     emissive buffer;
 //   * environment: an EXT_lights_image_based cube turned equirectangular, or
 //     an HDR file named in extras/extensions (Radiance RGBE or PNG, decoded as
 //     stb_image's stbi_loadf does), else the fallback sky.
-// Not supported, with an error rather than a different picture: JPEG, KTX2 and
-// WebP images (material textures and cube faces), interlaced PNGs, sparse
-// accessors, non-triangle primitives.  An environment HDR file in another
+// Not supported, with an error rather than a different picture: KTX2 and WebP
+// images (which stb_image, the reference's decoder, does not read either),
+// interlaced PNGs, sparse accessors, non-triangle primitives.  An environment HDR file in another
 // format leaves the fallback sky, as an undecodable one does in the reference.
 #include <zlib.h>
 
@@ -37,6 +38,7 @@
 #include <string>
 #include <vector>
 
+#include "jpeg.h"
 #include "json.h"
 #include "scene_impl.h"
 
@@ -356,10 +358,16 @@ bool decode_hdr(const std::vector<uint8_t>& f, FloatImage& img, std::string& err
   return true;
 }
 
+// An 8-bit image (PNG or JPEG) to RGBA8.
+bool decode_image(const std::vector<uint8_t>& f, HostTexture& t, std::string& err) {
+  if (is_jpeg(f)) return decode_jpeg(f, t, err);
+  return decode_png(f, t, err);
+}
+
 bool decode_float_image(const std::vector<uint8_t>& f, FloatImage& img, std::string& err) {
   if (is_hdr(f)) return decode_hdr(f, img, err);
   HostTexture t;
-  if (!decode_png(f, t, err)) return false;
+  if (!decode_image(f, t, err)) return false;
   img.w = t.width;
   img.h = t.height;
   img.rgba.resize(t.rgba8.size());
@@ -893,7 +901,8 @@ std::string load_gltf(const std::string& path, Scene::Impl& s) {
       std::vector<uint8_t> bytes;
       std::string mime;
       if (!image_bytes(d, *img, bytes, mime, err)) return "texture " + std::to_string(i) + ": " + err;
-      if (!decode_png(bytes, ht, err)) return "texture " + std::to_string(i) + ": " + err + (mime.empty() ? "" : " (" + mime + ")");
+      if (!decode_image(bytes, ht, err))
+        return "texture " + std::to_string(i) + ": " + err + (mime.empty() ? "" : " (" + mime + ")");
       s.textures.push_back(std::move(ht));
     }
   }

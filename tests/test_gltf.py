@@ -92,7 +92,8 @@ def _variant(tmp_path, mutate):
 
 
 @pytest.mark.parametrize("mutate,msg", [
-    (lambda d: d["images"].__setitem__(0, {"uri": "data:image/jpeg;base64,/9j/4AAQ"}), "not a PNG"),
+    (lambda d: d["images"].__setitem__(0, {"uri": "data:image/jpeg;base64,/9j/4AAQ"}), "JPEG"),
+    (lambda d: d["images"].__setitem__(0, {"uri": "data:image/webp;base64,UklGRg=="}), "not a PNG"),
     (lambda d: d["meshes"][1]["primitives"][0].__setitem__("mode", 1), "triangle"),
     (lambda d: d["meshes"][1]["primitives"][0].__setitem__("material", 7), "material"),
     (lambda d: d["nodes"][2].__setitem__("children", [9]), "node"),
