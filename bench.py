@@ -625,10 +625,15 @@ def main() -> int:
     env.sync()
     ms_s = ev.elapsed_ms(e0, e1) / args.steps
     env.set_rng_mode(env.RNG_REFERENCE)
+    alg_s = RAY_IO_BYTES * rays_s + NODE_BYTES * st_s["nodes"] + TRI_BYTES * st_s["tris"] + INST_BYTES * st_s["insts"]
     sample_mode = {"rng": "one stream per sample (CG_RNG_SAMPLE); images converge to the reference order's "
                           "(tests: 4096-spp relative MSE < 1e-3)",
                    "ms_per_frame": round(ms_s, 3), "Mrays/s": round(rays_s / (ms_s * 1e-3) / 1e6, 2),
-                   "rays_per_frame": rays_s, "speedup_vs_reference_order": round(ms / ms_s, 3)}
+                   "rays_per_frame": rays_s, "speedup_vs_reference_order": round(ms / ms_s, 3),
+                   "roofline": {"bound": "hbm", "kernel": "k_view_s (+ k_view_finish)",
+                                "achieved": round(alg_s / (ms_s * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                                "unit": "GB/s", "frac": round(alg_s / (ms_s * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                "algorithmic_bytes": alg_s}}
 
     # HBM bytes per launch come from the committed rocprofv3 --pmc runs of this
     # workload (FETCH_SIZE / WRITE_SIZE, scaled by a copy kernel of known size).
@@ -636,8 +641,14 @@ def main() -> int:
     prof = os.path.join(ROOT, "profiles", "k_view_c1_1280x720_16spp.json")
     if os.path.exists(prof):
         with open(prof) as f:
-            traffic = json.load(f)["traffic_bytes"]
+            pj = json.load(f)
+        traffic = pj["traffic_bytes"]
         traffic_src = os.path.relpath(prof, ROOT)
+        if "sample_stream" in pj:
+            t_s = pj["sample_stream"]["traffic_bytes"] + pj.get("sample_stream_finish", {}).get("traffic_bytes", 0)
+            sample_mode["roofline"]["traffic"] = round(t_s)
+            sample_mode["roofline"]["overfetch"] = round(t_s / alg_s, 3)
+            sample_mode["roofline"]["traffic_source"] = traffic_src
 
     micro = None
     if not args.no_traversal:
