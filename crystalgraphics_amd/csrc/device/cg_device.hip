@@ -255,6 +255,15 @@ __device__ __forceinline__ TStack lane_stack(uint32_t* s_ref, float* s_t, const 
 // shorten them.  The many-job kernel is bound by throughput and keeps 4 waves
 // per SIMD and 16 entries.
 constexpr int kViewShortStack = 24;
+// LDS stack entries per lane of the sample-stream kernels (k_view_s,
+// k_view_batch_s): their chains are short and their throughput needs waves.
+// They use 64 VGPRs, so LDS sets their occupancy: 12 entries (24 KB per
+// block, 6 waves per SIMD) beat 16 (5 waves) and 8 or 10 (more overflow
+// traffic) on the 64-job mix and the serving mix.
+#ifndef CG_SAMPLE_SHORT
+#define CG_SAMPLE_SHORT 12
+#endif
+constexpr int kSampleShortStack = CG_SAMPLE_SHORT;
 constexpr int kRefillLanes = 16;
 constexpr int kStepsPerRefill = 4;
 template <int kMode, class Fetch, class Done>
@@ -392,9 +401,9 @@ __global__ __launch_bounds__(kBlock, 3) void k_view(ViewArgs A, WorkQueue q, Sta
 template <int MF>
 __global__ __launch_bounds__(kBlock, 4) void k_view_s(ViewArgs A, SampleBufs B, uint32_t chunk, uint32_t nchunks,
                                                       WorkQueue q, StackArgs sa) {
-  __shared__ uint32_t s_ref[kShortStack * kBlock];
-  __shared__ float s_t[kShortStack * kBlock];
-  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kShortStack);
+  __shared__ uint32_t s_ref[kSampleShortStack * kBlock];
+  __shared__ float s_t[kSampleShortStack * kBlock];
+  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kSampleShortStack);
   const uint32_t tiles_x = (A.W + 7u) / 8u;
   TravStats st{0u, 0u, 0u};
   uint32_t nrays = 0, nshadow = 0;
@@ -472,9 +481,9 @@ __global__ __launch_bounds__(kBlock, 4) void k_view_batch_s(const ViewArgs* __re
                                                             const uint32_t* __restrict__ first, uint32_t njobs,
                                                             WorkQueue q, StackArgs sa,
                                                             unsigned long long* __restrict__ stats) {
-  __shared__ uint32_t s_ref[kShortStack * kBlock];
-  __shared__ float s_t[kShortStack * kBlock];
-  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kShortStack);
+  __shared__ uint32_t s_ref[kSampleShortStack * kBlock];
+  __shared__ float s_t[kSampleShortStack * kBlock];
+  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kSampleShortStack);
   TravStats st{0u, 0u, 0u};
   uint32_t nrays = 0, nshadow = 0;
   uint32_t probe = 0;
@@ -1002,7 +1011,7 @@ int launch_view_s_mf(cg_env* env, const ViewArgs& a, const SampleBufs& b, uint32
   if (items >= kNoItem) return fail(env, CG_ERR_INVALID, "render too large for the sample-stream mode");
   const uint32_t grid = resident_grid(k_view_s<MF>, env, static_cast<uint32_t>((items + 3u) / 4u));
   StackArgs sa;
-  int rc = stack_args(env, grid, depth, sa, kShortStack);
+  int rc = stack_args(env, grid, depth, sa, kSampleShortStack);
   if (rc != CG_OK) return rc;
   WorkQueue q = reset_queue(env, static_cast<uint32_t>(items));
   hipLaunchKernelGGL((k_view_s<MF>), dim3(grid), dim3(kBlock), 0, env->stream, a, b, chunk, nchunks, q, sa);
@@ -1791,13 +1800,19 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
       vs[k].srec = vs[k].path_hash ? rec + sample_off[k] : nullptr;
     }
   }
-  StackArgs sa{nullptr, 0, static_cast<uint32_t*>(env->errflag.p)};
-  if (depth > static_cast<uint32_t>(kShortStack)) {
-    sa.ovf_cap = static_cast<int>(depth) - kShortStack;
-    if ((rc = grow(env, v->ovf, static_cast<size_t>(std::max(grid, grid_s)) * kBlock * sa.ovf_cap * sizeof(uint2))) !=
-        CG_OK)
-      return give_back(rc);
-    sa.ovf = static_cast<uint2*>(v->ovf.p);
+  // overflow stacks: one area, sized for the larger need of the two kernels
+  StackArgs sa{nullptr, 0, static_cast<uint32_t*>(env->errflag.p)}, sa_s = sa;
+  const int cap = nref ? std::max(0, static_cast<int>(depth) - kShortStack) : 0;
+  const int cap_s = ns ? std::max(0, static_cast<int>(depth) - kSampleShortStack) : 0;
+  const size_t ovf_bytes = std::max(static_cast<size_t>(grid) * kBlock * cap, static_cast<size_t>(grid_s) * kBlock * cap_s) *
+                           sizeof(uint2);
+  if (ovf_bytes) {
+    if ((rc = grow(env, v->ovf, ovf_bytes)) != CG_OK) return give_back(rc);
+    sa.ovf = sa_s.ovf = static_cast<uint2*>(v->ovf.p);
+    sa.ovf_cap = cap;
+    sa_s.ovf_cap = cap_s;
+    if (!cap) sa.ovf = nullptr;
+    if (!cap_s) sa_s.ovf = nullptr;
   }
   v->host_jobs = std::move(va);
   v->host_first = std::move(first);
@@ -1842,13 +1857,13 @@ int cg_view_batch_launch(cg_env* env, const cg_job* jobs, uint32_t n, cg_vbatch*
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(kXcds * kCounterStride), 0, v->stream, ctr, kXcds * kCounterStride);
     const WorkQueue q{ctr, items_s, CG_VB_LPT ? 1u : 0u};
     if (frames_s <= 4)
-      launch_view_batch_s_mf<4>(v, grid_s, ns, q, sa, stats);
+      launch_view_batch_s_mf<4>(v, grid_s, ns, q, sa_s, stats);
     else if (frames_s <= 8)
-      launch_view_batch_s_mf<8>(v, grid_s, ns, q, sa, stats);
+      launch_view_batch_s_mf<8>(v, grid_s, ns, q, sa_s, stats);
     else if (frames_s <= 16)
-      launch_view_batch_s_mf<16>(v, grid_s, ns, q, sa, stats);
+      launch_view_batch_s_mf<16>(v, grid_s, ns, q, sa_s, stats);
     else
-      launch_view_batch_s_mf<64>(v, grid_s, ns, q, sa, stats);
+      launch_view_batch_s_mf<64>(v, grid_s, ns, q, sa_s, stats);
     hipLaunchKernelGGL(k_view_batch_finish, dim3((npix_s + kBlock - 1u) / kBlock), dim3(kBlock), 0, v->stream,
                        static_cast<const ViewArgs*>(v->jobs_s.p), static_cast<const uint32_t*>(v->pfirst_s.p), ns,
                        npix_s);
