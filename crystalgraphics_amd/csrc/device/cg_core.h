@@ -211,14 +211,14 @@ struct HitRec {
   uint32_t inst, slot;
 };
 
-// Traversal stack: the first `short_n` entries of each lane live in
-// `ref`/`t` (LDS, `stride`-interleaved), deeper entries in `ovf` (device
-// memory, `ovf_stride`-interleaved; sized from the tree's stack bound).  An
-// entry is a node ref and the distance at which the ray enters that node.
+// Traversal stack: the first `short_n` entries of each lane live in `lds`
+// (`stride`-interleaved, one 8-byte entry per access), deeper entries in
+// `ovf` (device memory, `ovf_stride`-interleaved; sized from the tree's stack
+// bound).  An entry is a node ref and the bits of the distance at which the
+// ray enters that node.
 constexpr int kShortStack = 16;  // default LDS entries per lane
 struct TStack {
-  uint32_t* ref;
-  float* t;
+  uint2* lds;
   int stride;
   int short_n;
   uint2* ovf;
@@ -227,8 +227,7 @@ struct TStack {
   uint32_t* err;  // set to 1 if an entry is dropped past ovf_cap (nullable)
   CG_HD void push(int& sp, uint32_t r, float tt) const {
     if (sp < short_n) {
-      ref[sp * stride] = r;
-      t[sp * stride] = tt;
+      lds[sp * stride] = make_uint2(r, __float_as_uint(tt));
     } else if (sp - short_n < ovf_cap) {
       ovf[static_cast<size_t>(sp - short_n) * ovf_stride] = make_uint2(r, __float_as_uint(tt));
     } else if (err) {
@@ -239,8 +238,9 @@ struct TStack {
   CG_HD void pop(int& sp, uint32_t& r, float& tt) const {
     --sp;
     if (sp < short_n) {
-      r = ref[sp * stride];
-      tt = t[sp * stride];
+      const uint2 e = lds[sp * stride];
+      r = e.x;
+      tt = __uint_as_float(e.y);
     } else if (sp - short_n < ovf_cap) {
       const uint2 e = ovf[static_cast<size_t>(sp - short_n) * ovf_stride];
       r = e.x;
@@ -351,16 +351,21 @@ CG_HD bool trav_step(const TravScene& S, TravRay& r, const TStack& stk, TravStat
     // closest hits are unchanged.
     const float ax = sx * r.inv.x, ay = sy * r.inv.y, az = sz * r.inv.z;
     const float bx = (ox - r.o.x) * r.inv.x, by = (oy - r.o.y) * r.inv.y, bz = (oz - r.o.z) * r.inv.z;
+    // q * a + b is monotone in q, rising where the ray's slope is positive:
+    // there the low bound is entered and the high bound left, elsewhere the
+    // reverse.  Choosing the bytes by the slope's sign once per node gives
+    // the same entry and exit distances as a min and a max per child.
+    const bool fx = ax >= 0.0f, fy = ay >= 0.0f, fz = az >= 0.0f;
+    const uint32_t nx0 = fx ? w1.x : w2.z, nx1 = fx ? w1.y : w2.w, mx0 = fx ? w2.z : w1.x, mx1 = fx ? w2.w : w1.y;
+    const uint32_t ny0 = fy ? w1.z : w3.x, ny1 = fy ? w1.w : w3.y, my0 = fy ? w3.x : w1.z, my1 = fy ? w3.y : w1.w;
+    const uint32_t nz0 = fz ? w2.x : w3.z, nz1 = fz ? w2.y : w3.w, mz0 = fz ? w3.z : w2.x, mz1 = fz ? w3.w : w2.y;
 #define CG_QB(w, i) static_cast<float>(((w) >> (8 * ((i) & 3))) & 0xFFu)
 #pragma unroll
     for (int i = 0; i < CG_WIDE; ++i) {
-      const uint32_t lx = i < 4 ? w1.x : w1.y, ly = i < 4 ? w1.z : w1.w, lz = i < 4 ? w2.x : w2.y;
-      const uint32_t hx = i < 4 ? w2.z : w2.w, hy = i < 4 ? w3.x : w3.y, hz = i < 4 ? w3.z : w3.w;
-      const float t0x = fmaf(CG_QB(lx, i), ax, bx), t1x = fmaf(CG_QB(hx, i), ax, bx);
-      const float t0y = fmaf(CG_QB(ly, i), ay, by), t1y = fmaf(CG_QB(hy, i), ay, by);
-      const float t0z = fmaf(CG_QB(lz, i), az, bz), t1z = fmaf(CG_QB(hz, i), az, bz);
-      const float tn = fmx(fmx(fmn(t0x, t1x), fmn(t0y, t1y)), fmn(t0z, t1z));
-      const float tf = fmn(fmn(fmx(t0x, t1x), fmx(t0y, t1y)), fmx(t0z, t1z));
+      const float tn = fmx(fmx(fmaf(CG_QB(i < 4 ? nx0 : nx1, i), ax, bx), fmaf(CG_QB(i < 4 ? ny0 : ny1, i), ay, by)),
+                           fmaf(CG_QB(i < 4 ? nz0 : nz1, i), az, bz));
+      const float tf = fmn(fmn(fmaf(CG_QB(i < 4 ? mx0 : mx1, i), ax, bx), fmaf(CG_QB(i < 4 ? my0 : my1, i), ay, by)),
+                           fmaf(CG_QB(i < 4 ? mz0 : mz1, i), az, bz));
       t[i] = (tf >= tn && tf > 0.0f && tn <= r.best.t && c[i] != CG_REF_NONE) ? tn : kInf;
     }
 #undef CG_QB

@@ -237,9 +237,9 @@ struct StackArgs {
 // The stack of this lane: `short_n` entries in the block's LDS arrays, the
 // rest in the overflow area interleaved over the whole grid.
 template <int BLOCK>
-__device__ __forceinline__ TStack lane_stack(uint32_t* s_ref, float* s_t, const StackArgs& sa, int short_n) {
+__device__ __forceinline__ TStack lane_stack(uint2* s_stk, const StackArgs& sa, int short_n) {
   const uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
-  return TStack{s_ref + threadIdx.x, s_t + threadIdx.x, BLOCK, short_n, sa.ovf ? sa.ovf + g : nullptr,
+  return TStack{s_stk + threadIdx.x, BLOCK, short_n, sa.ovf ? sa.ovf + g : nullptr,
                 gridDim.x * BLOCK, sa.ovf ? sa.ovf_cap : 0, sa.err};
 }
 
@@ -257,9 +257,9 @@ __device__ __forceinline__ TStack lane_stack(uint32_t* s_ref, float* s_t, const 
 constexpr int kViewShortStack = 24;
 // LDS stack entries per lane of the sample-stream kernels (k_view_s,
 // k_view_batch_s): their chains are short and their throughput needs waves.
-// They use 64 VGPRs, so LDS sets their occupancy: 12 entries (24 KB per
-// block, 6 waves per SIMD) beat 16 (5 waves) and 8 or 10 (more overflow
-// traffic) on the 64-job mix and the serving mix.
+// Their registers hold them to 4 waves per SIMD (128 VGPRs); within that, 12
+// entries (24 KB per block) beat 16, 10 and 8 on the 64-job mix and the
+// serving mix.
 #ifndef CG_SAMPLE_SHORT
 #define CG_SAMPLE_SHORT 12
 #endif
@@ -325,9 +325,8 @@ __global__ __launch_bounds__(kBlock, 5) void k_trace(DevScene S, const cg_ray* _
                                                   cg_hit* __restrict__ hits, float* __restrict__ uv,
                                                   uint32_t* __restrict__ occluded,
                                                   unsigned long long* __restrict__ stats, WorkQueue q, StackArgs sa) {
-  __shared__ uint32_t s_ref[kShortStack * kBlock];
-  __shared__ float s_t[kShortStack * kBlock];
-  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kShortStack);
+  __shared__ uint2 s_stk[kShortStack * kBlock];
+  const TStack stk = lane_stack<kBlock>(s_stk, sa, kShortStack);
   const TravScene scene = trav_scene(S);
   TravStats st{0u, 0u, 0u};
   uint64_t nq = 0;
@@ -376,9 +375,8 @@ __global__ __launch_bounds__(kBlock, 5) void k_trace(DevScene S, const cg_ray* _
 // (overflow in device memory).
 template <int MF>
 __global__ __launch_bounds__(kBlock, 3) void k_view(ViewArgs A, WorkQueue q, StackArgs sa) {
-  __shared__ uint32_t s_ref[kViewShortStack * kBlock];
-  __shared__ float s_t[kViewShortStack * kBlock];
-  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kViewShortStack);
+  __shared__ uint2 s_stk[kViewShortStack * kBlock];
+  const TStack stk = lane_stack<kBlock>(s_stk, sa, kViewShortStack);
   const uint32_t tiles_x = (A.W + 7u) / 8u;
   TravStats st{0u, 0u, 0u};
   uint32_t nrays = 0, nshadow = 0;  // per lane; summed in 64 bits at the flush
@@ -401,9 +399,8 @@ __global__ __launch_bounds__(kBlock, 3) void k_view(ViewArgs A, WorkQueue q, Sta
 template <int MF>
 __global__ __launch_bounds__(kBlock, 4) void k_view_s(ViewArgs A, SampleBufs B, uint32_t chunk, uint32_t nchunks,
                                                       WorkQueue q, StackArgs sa) {
-  __shared__ uint32_t s_ref[kSampleShortStack * kBlock];
-  __shared__ float s_t[kSampleShortStack * kBlock];
-  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kSampleShortStack);
+  __shared__ uint2 s_stk[kSampleShortStack * kBlock];
+  const TStack stk = lane_stack<kBlock>(s_stk, sa, kSampleShortStack);
   const uint32_t tiles_x = (A.W + 7u) / 8u;
   TravStats st{0u, 0u, 0u};
   uint32_t nrays = 0, nshadow = 0;
@@ -434,9 +431,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_view_batch(const ViewArgs* __rest
                                                           const uint32_t* __restrict__ first, uint32_t njobs,
                                                           WorkQueue q, StackArgs sa,
                                                           unsigned long long* __restrict__ stats) {
-  __shared__ uint32_t s_ref[kShortStack * kBlock];
-  __shared__ float s_t[kShortStack * kBlock];
-  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kShortStack);
+  __shared__ uint2 s_stk[kShortStack * kBlock];
+  const TStack stk = lane_stack<kBlock>(s_stk, sa, kShortStack);
   TravStats st{0u, 0u, 0u};
   uint32_t nrays = 0, nshadow = 0;  // per lane; summed in 64 bits at the flush
   uint32_t probe = 0;
@@ -481,9 +477,8 @@ __global__ __launch_bounds__(kBlock, 4) void k_view_batch_s(const ViewArgs* __re
                                                             const uint32_t* __restrict__ first, uint32_t njobs,
                                                             WorkQueue q, StackArgs sa,
                                                             unsigned long long* __restrict__ stats) {
-  __shared__ uint32_t s_ref[kSampleShortStack * kBlock];
-  __shared__ float s_t[kSampleShortStack * kBlock];
-  const TStack stk = lane_stack<kBlock>(s_ref, s_t, sa, kSampleShortStack);
+  __shared__ uint2 s_stk[kSampleShortStack * kBlock];
+  const TStack stk = lane_stack<kBlock>(s_stk, sa, kSampleShortStack);
   TravStats st{0u, 0u, 0u};
   uint32_t nrays = 0, nshadow = 0;
   uint32_t probe = 0;
@@ -646,9 +641,8 @@ __global__ __launch_bounds__(kBlock) void k_wf_admit(WfTables T, uint32_t j, uin
 // workgroup, SHORT LDS stack entries per lane (autotuned).
 template <int BLOCK, int SHORT>
 __global__ __launch_bounds__(BLOCK, 5) void k_wf_trace(WfTables T, uint32_t q, StackArgs sa) {
-  __shared__ uint32_t s_ref[SHORT * BLOCK];
-  __shared__ float s_t[SHORT * BLOCK];
-  const TStack stk = lane_stack<BLOCK>(s_ref, s_t, sa, SHORT);
+  __shared__ uint2 s_stk[SHORT * BLOCK];
+  const TStack stk = lane_stack<BLOCK>(s_stk, sa, SHORT);
   const uint32_t n = T.counters[kWfQueueLen + 32u * q];
   const WorkQueue wq{T.counters + kWfTraceWork, (n + 63u) / 64u};
   const uint32_t* queue = T.queue[q];
@@ -684,9 +678,8 @@ __global__ __launch_bounds__(BLOCK, 5) void k_wf_trace(WfTables T, uint32_t q, S
 // waves spill, and the 64-job mix runs 5 % slower).
 template <int BLOCK, int SHORT>
 __global__ __launch_bounds__(BLOCK, 3) void k_wf_shade(WfTables T, uint32_t q, StackArgs sa) {
-  __shared__ uint32_t s_ref[SHORT * BLOCK];
-  __shared__ float s_t[SHORT * BLOCK];
-  const TStack stk = lane_stack<BLOCK>(s_ref, s_t, sa, SHORT);
+  __shared__ uint2 s_stk[SHORT * BLOCK];
+  const TStack stk = lane_stack<BLOCK>(s_stk, sa, SHORT);
   const uint32_t n = T.counters[kWfQueueLen + 32u * q];
   const WorkQueue wq{T.counters + kWfShadeWork, (n + 63u) / 64u};
   const uint32_t* queue = T.queue[q];
