@@ -242,8 +242,8 @@ def batched_form(cg, seed: int, n_jobs: int, budget_gib: float, log, tune_engine
     # profilers that trace graph launches see a single instantiation)
     tune = autotune.tune_batch(env, scenes, jobs, depth, log=log) if tune_engine else autotune.BatchTuning()
     tune_s = time.perf_counter() - t0
-    need = sum(((j.width + 7) // 8) * ((j.height + 7) // 8) * 64 for j in jobs)
-    tune.slot_capacity = autotune.capacity_for(int(budget_gib * (1 << 30)), depth, need)
+    # the pool the tuner chose, as a fraction of what the jobs could keep resident, within the budget
+    tune.slot_capacity = tune.capacity(jobs, depth, int(budget_gib * (1 << 30)))
     rjs = [cg.RenderJob(env, scenes[j.scene], j.camera(), j.conf()) for j in jobs]
     env.sync()
     alone, t0 = [], time.perf_counter()
@@ -255,8 +255,8 @@ def batched_form(cg, seed: int, n_jobs: int, budget_gib: float, log, tune_engine
     t_alone = time.perf_counter() - t0
     ref = [hashlib.sha1(rj.read_history().tobytes()).hexdigest() for rj in rjs]
     log(f"one job at a time: {t_alone:.2f} s")
-    # All jobs in one persistent launch of the per-pixel kernel, largest first.
-    order = sorted(range(len(jobs)), key=lambda i: -jobs[i].pixels * jobs[i].spp * jobs[i].depth)
+    # All jobs in one persistent launch of the per-pixel kernel, in the tile order the tuner chose.
+    order = autotune.view_order(jobs, tune.view_order)
     env.sync()
     t0 = time.perf_counter()
     vb = cg.ViewBatch(env, [rjs[i] for i in order])
@@ -316,19 +316,26 @@ def batched_form(cg, seed: int, n_jobs: int, budget_gib: float, log, tune_engine
         rj.free()
     env.close()
     lat = list(done_at.values())
+    wf_t = [t for p, t in tune.trials[1:] if "view_order" not in p]
     seq = list(np.cumsum(alone))
     pct = workloads.percentile
     return {
         "workload": f"{n_jobs} jobs, resolution log-uniform 64x64..1920x1080, spp {{1,4,16,64}}, depth 2-12, "
                     f"over 4 resident scenes of {', '.join(f'{t/1e6:.2f}M' for t in tris)} tris",
         "pixels": sum(j.pixels for j in jobs), "rays": rays, "setup_s": round(setup, 1),
-        "autotune": {"seconds": round(tune_s, 1), "chosen": tune.kwargs(), "trials": len(tune.trials)},
+        "autotune": {"seconds": round(tune_s, 1),
+                     "chosen": dict(tune.kwargs(), slot_fraction=tune.slot_fraction, view_order=tune.view_order),
+                     "trials": len(tune.trials),
+                     # the wavefront engine's calibration times (the first trial is a warm-up)
+                     "wavefront_trial_ms": {"min": round(min(wf_t) * 1e3, 1), "max": round(max(wf_t) * 1e3, 1)}
+                     if wf_t else None,
+                     "view_order_ms": {p["view_order"]: round(t * 1e3, 1) for p, t in tune.trials if "view_order" in p}},
         "one_at_a_time": {"seconds": round(t_alone, 3), "Mrays/s": round(rays / t_alone / 1e6, 1),
                           "job_latency_ms": {"p50": round(pct(seq, 50) * 1e3, 1), "p99": round(pct(seq, 99) * 1e3, 1)},
                           "job_render_ms": {"p50": round(pct(alone, 50) * 1e3, 1),
                                             "p99": round(pct(alone, 99) * 1e3, 1)}},
         "batched": {"engine": "one persistent launch of the per-pixel kernel over all jobs' tiles "
-                              "(cg_view_batch, largest job first)",
+                              f"(cg_view_batch, tile order {tune.view_order}, autotuned)",
                     "seconds": round(t_mega, 3), "Mrays/s": round(rays / t_mega / 1e6, 1),
                     "identical_jobs": f"{same_mega}/{len(rjs)}"},
         "batched_wavefront": {"engine": "wavefront rounds over HBM-resident path state, graph-captured "

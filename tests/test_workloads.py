@@ -66,6 +66,30 @@ def test_slot_sizing():
     assert autotune.capacity_for(64 << 30, 12, 3_000_000) == 3_000_000
 
 
+def test_pool_from_the_chosen_fraction():
+    jobs = [workloads.JobSpec(0, 100, 50, 4, 8, (3, 0, 0), (-1, 0, 0)),
+            workloads.JobSpec(1, 64, 64, 1, 2, (0, 3, 0), (0, -1, 0))]
+    # 100x50 -> 13x7 tiles, 64x64 -> 8x8 tiles, 64 slots per tile
+    assert autotune.resident_pixels(jobs) == (13 * 7 + 8 * 8) * 64
+    t = autotune.BatchTuning(slot_fraction=0.5)
+    assert t.capacity(jobs, 8, 64 << 30) == 1 << 16  # half of 9920 is under the floor
+    big = [workloads.JobSpec(0, 1920, 1080, 1, 8, (3, 0, 0), (-1, 0, 0))] * 8
+    need = autotune.resident_pixels(big)
+    assert t.capacity(big, 8, 64 << 30) == need // 2
+    assert t.capacity(big, 8, 1 << 30) == (1 << 30) // autotune.slot_bytes(8)  # the budget binds
+
+
+def test_view_order():
+    jobs = workloads.batch_jobs(16, 4, seed=3)
+    order = autotune.view_order(jobs, "largest_first")
+    assert sorted(order) == list(range(16))
+    work = [jobs[i].pixels * jobs[i].spp * jobs[i].depth for i in order]
+    assert work == sorted(work, reverse=True)
+    assert autotune.view_order(jobs, "as_given") == list(range(16))
+    with pytest.raises(ValueError):
+        autotune.view_order(jobs, "random")
+
+
 @pytest.mark.parametrize("q", [0, 1, 37.5, 50, 99, 100])
 def test_percentile_matches_numpy(q):
     v = list(np.random.default_rng(2).random(101))
