@@ -335,11 +335,12 @@ def batched_form(cg, seed: int, n_jobs: int, budget_gib: float, log, tune_engine
                           "job_render_ms": {"p50": round(pct(alone, 50) * 1e3, 1),
                                             "p99": round(pct(alone, 99) * 1e3, 1)}},
         "batched": {"engine": "one persistent launch of the per-pixel kernel over all jobs' tiles "
-                              f"(cg_view_batch, tile order {tune.view_order}, autotuned)",
+                              f"(cg_view_batch, tile order {tune.view_order}{', autotuned' if tune.trials else ''})",
                     "seconds": round(t_mega, 3), "Mrays/s": round(rays / t_mega / 1e6, 1),
                     "identical_jobs": f"{same_mega}/{len(rjs)}"},
         "batched_wavefront": {"engine": "wavefront rounds over HBM-resident path state, graph-captured "
-                                        "(cg_batch, autotuned), continuous admission",
+                                        f"(cg_batch{', autotuned' if tune.trials else ', default shape'}), "
+                                        "continuous admission",
                               "seconds": round(t_batch, 3), "Mrays/s": round(rays / t_batch / 1e6, 1),
                               "launches": launches,
                               "job_latency_ms": {"p50": round(pct(lat, 50) * 1e3, 1),
