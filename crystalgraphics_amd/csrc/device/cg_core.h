@@ -959,19 +959,14 @@ CG_HD float light_pdf_linear(const DevScene& S, f3 rd, f3 pos) {
   }
   return pdf;
 }
-// The same sum, with the emitters the ray passes through found by a BVH over
-// the emitters (padded boxes, no distance cut-off) and added in ascending
-// emitter index, so the result equals the linear loop's bit for bit.  More
-// than kCap hits fall back to the linear loop.
-CG_HD float light_pdf(const DevScene& S, f3 next, f3 pos, const TStack& stk) {
-  if (S.alias_count == 0u) return 0.0f;
-  const f3 rd = normalize(next);
-  if (S.emit_root == CG_REF_NONE) return light_pdf_linear(S, rd, pos);
-  constexpr int kCap = 8;
-  uint32_t idx[kCap];
-  float hds[kCap];
-  int n = 0;
-  const f3 inv = dir_inv(rd);
+// The emitter-BVH walk of light_pdf: among the emitters the ray passes
+// through (padded boxes, no distance cut-off) with index above `after`, the
+// one of smallest index (CG_MISS_ID: none) and its distance, and how many
+// there are.
+CG_HD uint32_t emit_next_hit(const DevScene& S, f3 rd, f3 inv, f3 pos, uint32_t after, float& hd_out, uint32_t& count,
+                             const TStack& stk) {
+  uint32_t best = CG_MISS_ID;
+  count = 0u;
   uint32_t ref = S.emit_root;
   int sp = 0;
   for (;;) {
@@ -993,29 +988,46 @@ CG_HD float light_pdf(const DevScene& S, f3 next, f3 pos, const TStack& stk) {
       const uint32_t cnt = ((ref >> CG_REF_COUNT_SHIFT) & 3u) + 1u;
       for (uint32_t i = 0; i < cnt; ++i) {
         const uint32_t e = S.emit_order[slot + i];
+        if (after != CG_MISS_ID && e <= after) continue;
         const float hd = emit_hit_dist(pos, rd, S.emit[e]);
         if (hd <= 0.0f) continue;
-        if (n == kCap) return light_pdf_linear(S, rd, pos);
-        int k = n++;
-        while (k > 0 && idx[k - 1] > e) {
-          idx[k] = idx[k - 1];
-          hds[k] = hds[k - 1];
-          --k;
+        ++count;
+        if (e < best) {
+          best = e;
+          hd_out = hd;
         }
-        idx[k] = e;
-        hds[k] = hd;
       }
     }
     if (sp == 0) break;
     float unused;
     stk.pop(sp, ref, unused);
   }
+  return best;
+}
+// The same sum, with the emitters the ray passes through found by the BVH
+// over the emitters and added in ascending emitter index, so the result
+// equals the linear loop's bit for bit: one walk finds how many emitters the
+// ray passes and the first of them; a ray through several walks again for
+// each next one (rare: emitters are small), holding no list per lane.
+CG_HD float light_pdf(const DevScene& S, f3 next, f3 pos, const TStack& stk) {
+  if (S.alias_count == 0u) return 0.0f;
+  const f3 rd = normalize(next);
+  if (S.emit_root == CG_REF_NONE) return light_pdf_linear(S, rd, pos);
+  const f3 inv = dir_inv(rd);
   float pdf = 0.0f;
-  for (int k = 0; k < n; ++k) {
-    const EmitRec& e = S.emit[idx[k]];
-    f3 ln = mk3(e.ln_x, e.ln_y, e.ln_z);
+  uint32_t after = CG_MISS_ID, count = 0u;
+  float hd = 0.0f;
+  uint32_t e = emit_next_hit(S, rd, inv, pos, after, hd, count, stk);
+  for (uint32_t k = 0; k < count; ++k) {
+    if (k > 0u) {
+      uint32_t unused;
+      e = emit_next_hit(S, rd, inv, pos, after, hd, unused, stk);
+    }
+    const EmitRec& er = S.emit[e];
+    f3 ln = mk3(er.ln_x, er.ln_y, er.ln_z);
     float lp = CG_FABS(dot(ln, -rd));
-    pdf += e.area_pdf * hds[k] * hds[k] / fmx(lp, 1e-3f);
+    pdf += er.area_pdf * hd * hd / fmx(lp, 1e-3f);
+    after = e;
   }
   return pdf;
 }
