@@ -526,32 +526,33 @@ static const SmitsTables kSmits = {
      {0.0001f, 0.0f, 0.1088f, 0.6651f, 1.0f, 1.0f, 0.9996f, 0.9586f, 0.9685f, 0.984f}}};
 enum { kWhite = 0, kRed, kGreen, kBlue, kMagenta, kCyan, kYellow };
 
-// The seven curves share the wavelength's bin and mix weight.
+// The seven curves share the wavelength's bin and mix weight.  A wavelength
+// at or beyond an end of the table reads that end's sample with weight 0:
+// (1 - 0) * c + 0 * c' = c exactly for the table's finite values.
 struct SmitsBin {
-  uint32_t b0, b1;
+  uint32_t b0;  // lower sample (0..9); the upper one is min(b0 + 1, 9)
   float mix;
-  int edge;  // -1: l <= 380, +1: l >= 720, 0 interior
 };
 CG_HD SmitsBin smits_bin(float l) {
-  SmitsBin b{0u, 0u, 0.0f, 0};
-  if (l <= 380.0f) {
-    b.edge = -1;
-    return b;
-  }
-  if (l >= 720.0f) {
-    b.edge = 1;
-    return b;
-  }
-  uint32_t bin = f2u(10.0f * (l - 380.0f) / 340.0f);
-  b.b0 = umin(bin, 9u);
-  b.b1 = umin(bin + 1u, 9u);
-  b.mix = (l - kSmits.bins[b.b0]) / (kSmits.bins[b.b1] - kSmits.bins[b.b0]);
-  return b;
+  if (l <= 380.0f) return SmitsBin{0u, 0.0f};
+  if (l >= 720.0f) return SmitsBin{9u, 0.0f};
+  const uint32_t bin = f2u(10.0f * (l - 380.0f) / 340.0f);
+  const uint32_t b0 = umin(bin, 9u), b1 = umin(bin + 1u, 9u);
+  return SmitsBin{b0, (l - kSmits.bins[b0]) / (kSmits.bins[b1] - kSmits.bins[b0])};
 }
-CG_HD float smits_at(int c, const SmitsBin& b) {
-  if (b.edge < 0) return kSmits.curve[c][0];
-  if (b.edge > 0) return kSmits.curve[c][9];
-  return (1.0f - b.mix) * kSmits.curve[c][b.b0] + b.mix * kSmits.curve[c][b.b1];
+// The curves, copied to LDS by every kernel that shades (smits_lds_init) so
+// a lookup is one 32-bit LDS address per lane, not a pointer held per lane.
+__shared__ float cg_smits_lds[7 * 10];
+__device__ __forceinline__ void smits_lds_init() {
+  for (uint32_t i = threadIdx.x; i < 70u; i += blockDim.x) cg_smits_lds[i] = kSmits.curve[i / 10u][i % 10u];
+  __syncthreads();
+}
+// (Read through a volatile pointer: the reads stay where they are used
+// instead of being hoisted into registers held across the sample.)
+__device__ __forceinline__ float smits_at(int c, const SmitsBin& b) {
+  const volatile float* t = cg_smits_lds + c * 10;
+  const uint32_t b1 = umin(b.b0 + 1u, 9u);
+  return (1.0f - b.mix) * t[b.b0] + b.mix * t[b1];
 }
 // material.wesl:264-297 / environment.wesl:27-61 / pdf/emission.wesl:84-117.
 CG_HD float rgb_lift(f3 rgb, const SmitsBin& b) {

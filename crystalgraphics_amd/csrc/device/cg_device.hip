@@ -257,8 +257,9 @@ __device__ __forceinline__ TStack lane_stack(uint2* s_stk, const StackArgs& sa, 
 constexpr int kViewShortStack = 24;
 // LDS stack entries per lane of the sample-stream kernels (k_view_s,
 // k_view_batch_s): their chains are short and their throughput needs waves.
-// Their registers hold them to 4 waves per SIMD (128 VGPRs); within that, 12
-// entries (24 KB per block) beat 16, 10 and 8 on the 64-job mix and the
+// They run 5 waves per SIMD (96 VGPRs, some spills: 5 waves beat 4 by 3 % on
+// one frame and 7 % on the 64-job mix; 3 waves lose 18 %); 12 entries (24 KB
+// per block, five blocks per CU) beat 16, 10 and 8 on the 64-job mix and the
 // serving mix.
 #ifndef CG_SAMPLE_SHORT
 #define CG_SAMPLE_SHORT 12
@@ -376,6 +377,7 @@ __global__ __launch_bounds__(kBlock, 5) void k_trace(DevScene S, const cg_ray* _
 template <int MF>
 __global__ __launch_bounds__(kBlock, 3) void k_view(ViewArgs A, WorkQueue q, StackArgs sa) {
   __shared__ uint2 s_stk[kViewShortStack * kBlock];
+  smits_lds_init();
   const TStack stk = lane_stack<kBlock>(s_stk, sa, kViewShortStack);
   const uint32_t tiles_x = (A.W + 7u) / 8u;
   TravStats st{0u, 0u, 0u};
@@ -397,9 +399,10 @@ __global__ __launch_bounds__(kBlock, 3) void k_view(ViewArgs A, WorkQueue q, Sta
 // several waves at once and no wave carries a long chain of dependent
 // queries; the per-sample results go to B and k_view_finish sums them.
 template <int MF>
-__global__ __launch_bounds__(kBlock, 4) void k_view_s(ViewArgs A, SampleBufs B, uint32_t chunk, uint32_t nchunks,
+__global__ __launch_bounds__(kBlock, 5) void k_view_s(ViewArgs A, SampleBufs B, uint32_t chunk, uint32_t nchunks,
                                                       WorkQueue q, StackArgs sa) {
   __shared__ uint2 s_stk[kSampleShortStack * kBlock];
+  smits_lds_init();
   const TStack stk = lane_stack<kBlock>(s_stk, sa, kSampleShortStack);
   const uint32_t tiles_x = (A.W + 7u) / 8u;
   TravStats st{0u, 0u, 0u};
@@ -432,6 +435,7 @@ __global__ __launch_bounds__(kBlock, 4) void k_view_batch(const ViewArgs* __rest
                                                           WorkQueue q, StackArgs sa,
                                                           unsigned long long* __restrict__ stats) {
   __shared__ uint2 s_stk[kShortStack * kBlock];
+  smits_lds_init();
   const TStack stk = lane_stack<kBlock>(s_stk, sa, kShortStack);
   TravStats st{0u, 0u, 0u};
   uint32_t nrays = 0, nshadow = 0;  // per lane; summed in 64 bits at the flush
@@ -473,11 +477,12 @@ __device__ __forceinline__ uint32_t job_of_item(const uint32_t* __restrict__ fir
 // Sample-stream jobs of a many-job launch: the items are every job's (tile,
 // sample chunk) pairs; k_view_batch_finish then sums each pixel's samples.
 template <int MF>
-__global__ __launch_bounds__(kBlock, 4) void k_view_batch_s(const ViewArgs* __restrict__ jobs,
+__global__ __launch_bounds__(kBlock, 5) void k_view_batch_s(const ViewArgs* __restrict__ jobs,
                                                             const uint32_t* __restrict__ first, uint32_t njobs,
                                                             WorkQueue q, StackArgs sa,
                                                             unsigned long long* __restrict__ stats) {
   __shared__ uint2 s_stk[kSampleShortStack * kBlock];
+  smits_lds_init();
   const TStack stk = lane_stack<kBlock>(s_stk, sa, kSampleShortStack);
   TravStats st{0u, 0u, 0u};
   uint32_t nrays = 0, nshadow = 0;
@@ -679,6 +684,7 @@ __global__ __launch_bounds__(BLOCK, 5) void k_wf_trace(WfTables T, uint32_t q, S
 template <int BLOCK, int SHORT>
 __global__ __launch_bounds__(BLOCK, 3) void k_wf_shade(WfTables T, uint32_t q, StackArgs sa) {
   __shared__ uint2 s_stk[SHORT * BLOCK];
+  smits_lds_init();
   const TStack stk = lane_stack<BLOCK>(s_stk, sa, SHORT);
   const uint32_t n = T.counters[kWfQueueLen + 32u * q];
   const WorkQueue wq{T.counters + kWfShadeWork, (n + 63u) / 64u};
