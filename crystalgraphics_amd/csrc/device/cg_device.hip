@@ -394,6 +394,26 @@ __global__ __launch_bounds__(kBlock, 3) void k_view(ViewArgs A, WorkQueue q, Sta
   if (A.stats) flush_stats(A.stats, nrays, nshadow, st.nodes, st.tris, st.insts);
 }
 
+// The lane task of a sample-stream work item: item c of a tile covers lane
+// tasks t = 64c .. 64c + 63 of the tile's 64 x nchunks (pixel, chunk)
+// pairs, taken pixel-major, so a wave holds few pixels and many of their
+// samples (at 16 spp, a 2x2 block x 16 samples): one pixel's samples share
+// the camera ray and the first surface, and their paths are alike in length.
+// Pixels are in Morton order within the tile.  The image does not depend on
+// the mapping (every sample draws from its own stream).
+struct SampleTask {
+  uint32_t px, py, s0, s1;
+};
+__device__ __forceinline__ SampleTask sample_task(uint32_t tile, uint32_t c, uint32_t lane, uint32_t tiles_x,
+                                                  uint32_t nchunks, uint32_t chunk, uint32_t spp) {
+  const uint32_t t = c * 64u + lane;
+  const uint32_t p = t / nchunks, q = t - p * nchunks;
+  const uint32_t mx = (p & 1u) | ((p >> 1) & 2u) | ((p >> 2) & 4u);
+  const uint32_t my = ((p >> 1) & 1u) | ((p >> 2) & 2u) | ((p >> 3) & 4u);
+  const uint32_t s0 = q * chunk;
+  return SampleTask{(tile % tiles_x) * 8u + mx, (tile / tiles_x) * 8u + my, s0, s0 + chunk < spp ? s0 + chunk : spp};
+}
+
 // Sample-stream mode: the work items are (8x8 tile, chunk of `chunk`
 // samples) pairs, a tile's chunks adjacent, so one pixel's samples run on
 // several waves at once and no wave carries a long chain of dependent
@@ -413,10 +433,8 @@ __global__ __launch_bounds__(kBlock, 5) void k_view_s(ViewArgs A, SampleBufs B, 
     const uint32_t item = grab_item(q, probe);
     if (item == kNoItem) break;
     const uint32_t tile = item / nchunks, c = item - tile * nchunks;
-    const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u);
-    const uint32_t py = (tile / tiles_x) * 8u + (lane >> 3);
-    const uint32_t s0 = c * chunk, s1 = s0 + chunk < A.spp ? s0 + chunk : A.spp;
-    if (px < A.W && py < A.H) render_samples<MF>(A, B, px, py, s0, s1, stk, st, nrays, nshadow);
+    const SampleTask k = sample_task(tile, c, lane, tiles_x, nchunks, chunk, A.spp);
+    if (k.px < A.W && k.py < A.H) render_samples<MF>(A, B, k.px, k.py, k.s0, k.s1, stk, st, nrays, nshadow);
   }
   if (A.stats) flush_stats(A.stats, nrays, nshadow, st.nodes, st.tris, st.insts);
 }
@@ -495,11 +513,9 @@ __global__ __launch_bounds__(kBlock, 5) void k_view_batch_s(const ViewArgs* __re
     const ViewArgs& A = jobs[j];
     const uint32_t local = item - first[j];
     const uint32_t tile = local / A.nchunks, c = local - tile * A.nchunks;
-    const uint32_t tiles_x = (A.W + 7u) / 8u;
-    const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u);
-    const uint32_t py = (tile / tiles_x) * 8u + (lane >> 3);
-    const uint32_t s0 = c * A.chunk, s1 = s0 + A.chunk < A.spp ? s0 + A.chunk : A.spp;
-    if (px < A.W && py < A.H) render_samples<MF>(A, SampleBufs{A.sxyz, A.srec}, px, py, s0, s1, stk, st, nrays, nshadow);
+    const SampleTask k = sample_task(tile, c, lane, (A.W + 7u) / 8u, A.nchunks, A.chunk, A.spp);
+    if (k.px < A.W && k.py < A.H)
+      render_samples<MF>(A, SampleBufs{A.sxyz, A.srec}, k.px, k.py, k.s0, k.s1, stk, st, nrays, nshadow);
   }
   if (stats) flush_stats(stats, nrays, nshadow, st.nodes, st.tris, st.insts);
 }
