@@ -1015,10 +1015,11 @@ int launch_view_ref(cg_env* env, const ViewArgs& a, int frames, uint32_t tiles, 
   if (frames <= 16) return launch_view_mf<16>(env, a, tiles, depth);
   return launch_view_mf<64>(env, a, tiles, depth);
 }
-// Samples per work item of the sample-stream mode: one sample per item up to
-// 16 samples per pixel, then enough that a tile has at most 16 items (the
-// image does not depend on it: samples are summed in sample order).
-uint32_t sample_chunk(uint32_t spp) { return spp <= 16u ? 1u : (spp + 15u) / 16u; }
+// Samples per lane of the sample-stream mode: one up to 64 samples per pixel
+// (a 64-spp wave is one pixel's samples), then enough that a tile has at
+// most 64 items (the image does not depend on it: samples are summed in
+// sample order).
+uint32_t sample_chunk(uint32_t spp) { return spp <= 64u ? 1u : (spp + 63u) / 64u; }
 template <int MF>
 int launch_view_s_mf(cg_env* env, const ViewArgs& a, const SampleBufs& b, uint32_t tiles, uint32_t depth) {
   const uint32_t chunk = sample_chunk(a.spp), nchunks = (a.spp + chunk - 1u) / chunk;
