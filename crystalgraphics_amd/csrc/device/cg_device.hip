@@ -414,9 +414,9 @@ __device__ __forceinline__ SampleTask sample_task(uint32_t tile, uint32_t c, uin
   return SampleTask{(tile % tiles_x) * 8u + mx, (tile / tiles_x) * 8u + my, s0, s0 + chunk < spp ? s0 + chunk : spp};
 }
 
-// Sample-stream mode: the work items are (8x8 tile, chunk of `chunk`
-// samples) pairs, a tile's chunks adjacent, so one pixel's samples run on
-// several waves at once and no wave carries a long chain of dependent
+// Sample-stream mode: a tile's 64 pixels x nchunks chunks of `chunk` samples
+// are nchunks work items of 64 lane tasks (sample_task), so a pixel's samples
+// run on separate lanes and no lane carries a long chain of dependent
 // queries; the per-sample results go to B and k_view_finish sums them.
 template <int MF>
 __global__ __launch_bounds__(kBlock, 5) void k_view_s(ViewArgs A, SampleBufs B, uint32_t chunk, uint32_t nchunks,
