@@ -1009,16 +1009,19 @@ CG_HD uint32_t emit_next_hit(const DevScene& S, f3 rd, f3 inv, f3 pos, uint32_t 
 // over the emitters and added in ascending emitter index, so the result
 // equals the linear loop's bit for bit: one walk finds how many emitters the
 // ray passes and the first of them; a ray through several walks again for
-// each next one (rare: emitters are small), holding no list per lane.
+// each next one (rare: emitters are small), holding no list per lane.  A ray
+// through more than kWalks emitters takes the linear loop (the same sum).
 CG_HD float light_pdf(const DevScene& S, f3 next, f3 pos, const TStack& stk) {
   if (S.alias_count == 0u) return 0.0f;
   const f3 rd = normalize(next);
   if (S.emit_root == CG_REF_NONE) return light_pdf_linear(S, rd, pos);
+  constexpr uint32_t kWalks = 16u;
   const f3 inv = dir_inv(rd);
   float pdf = 0.0f;
   uint32_t after = CG_MISS_ID, count = 0u;
   float hd = 0.0f;
   uint32_t e = emit_next_hit(S, rd, inv, pos, after, hd, count, stk);
+  if (count > kWalks) return light_pdf_linear(S, rd, pos);
   for (uint32_t k = 0; k < count; ++k) {
     if (k > 0u) {
       uint32_t unused;
