@@ -334,7 +334,13 @@ CG_HD bool trav_step(const TravScene& S, TravRay& r, const TStack& stk, TravStat
   const bool wide = at_node || (at_leaf && cnt >= 2u);  // six words
   const uint4* p3 = (wide || !at_leaf) ? p + 3 : p;
   const uint4* p5 = wide ? p + 5 : p;
-  const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p3[0], w4 = p3[1], w5 = p5[0];
+  uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p3[0], w4 = p3[1], w5 = p5[0];
+  // All six words arrive as six 16-byte loads issued together: left to
+  // itself the compiler narrows some to single words and re-reads them inside
+  // the branch that uses them, a second memory round trip per visit.
+  asm volatile("" : "+v"(w0.x), "+v"(w0.y), "+v"(w0.z), "+v"(w0.w), "+v"(w1.x), "+v"(w1.y), "+v"(w1.z), "+v"(w1.w),
+               "+v"(w2.x), "+v"(w2.y), "+v"(w2.z), "+v"(w2.w), "+v"(w3.x), "+v"(w3.y), "+v"(w3.z), "+v"(w3.w),
+               "+v"(w4.x), "+v"(w4.y), "+v"(w4.z), "+v"(w4.w), "+v"(w5.x), "+v"(w5.y), "+v"(w5.z), "+v"(w5.w));
   if (at_node) {
     // 128-B quantised node: origin + exponents, 8-bit child bounds
     // (lo x/y/z, hi x/y/z; 8 bytes each, byte i = child i), refs.
