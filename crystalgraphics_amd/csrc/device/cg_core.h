@@ -978,16 +978,25 @@ CG_HD uint32_t emit_next_hit(const DevScene& S, f3 rd, f3 inv, f3 pos, uint32_t 
   int sp = 0;
   for (;;) {
     if ((ref & CG_REF_LEAF) == 0u) {
-      const PairNode* nd = S.nodes + ref;
-      const bool hl = box_enter(pos, inv, nd->lo_l, nd->hi_l) < __builtin_inff();
-      const bool hr = box_enter(pos, inv, nd->lo_r, nd->hi_r) < __builtin_inff();
+      // the 64-byte node as four 16-byte loads issued together (see trav_step)
+      const uint4* np = reinterpret_cast<const uint4*>(S.nodes + ref);
+      uint4 a = np[0], b = np[1], c = np[2], d = np[3];
+      asm volatile("" : "+v"(a.x), "+v"(a.y), "+v"(a.z), "+v"(a.w), "+v"(b.x), "+v"(b.y), "+v"(b.z), "+v"(b.w),
+                   "+v"(c.x), "+v"(c.y), "+v"(c.z), "+v"(c.w), "+v"(d.x), "+v"(d.y));
+      // PairNode: lo_l (a.xyz), hi_l (a.w, b.xy), lo_r (b.zw, c.x), hi_r (c.yzw), ref_l, ref_r (d.xy)
+      const float lo_l[3] = {__uint_as_float(a.x), __uint_as_float(a.y), __uint_as_float(a.z)};
+      const float hi_l[3] = {__uint_as_float(a.w), __uint_as_float(b.x), __uint_as_float(b.y)};
+      const float lo_r[3] = {__uint_as_float(b.z), __uint_as_float(b.w), __uint_as_float(c.x)};
+      const float hi_r[3] = {__uint_as_float(c.y), __uint_as_float(c.z), __uint_as_float(c.w)};
+      const bool hl = box_enter(pos, inv, lo_l, hi_l) < __builtin_inff();
+      const bool hr = box_enter(pos, inv, lo_r, hi_r) < __builtin_inff();
       if (hl && hr) {
-        stk.push(sp, nd->ref_r, 0.0f);
-        ref = nd->ref_l;
+        stk.push(sp, d.y, 0.0f);
+        ref = d.x;
         continue;
       }
       if (hl || hr) {
-        ref = hl ? nd->ref_l : nd->ref_r;
+        ref = hl ? d.x : d.y;
         continue;
       }
     } else {
