@@ -459,7 +459,7 @@ class Batch:
 
     def __init__(self, env: DeviceEnv, slot_capacity: int = 1 << 20, max_trace_depth: int = 8,
                  rounds_per_launch: int = 16, use_graph: bool = True, max_jobs: int = 256,
-                 max_stack_depth: int = 64, block_size: int = 256, short_stack: int = 16):
+                 max_stack_depth: int = 128, block_size: int = 256, short_stack: int = 16):
         self.env = env
         self.conf = native.BatchConf(slot_capacity, max_trace_depth, rounds_per_launch, int(use_graph), max_jobs,
                                      max_stack_depth, block_size, short_stack)

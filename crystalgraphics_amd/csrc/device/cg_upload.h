@@ -663,7 +663,9 @@ inline uint32_t emit_pairs(const SahTree& t, std::vector<PairNode>& out, uint32_
 }
 
 // Rebuilt acceleration structure (same closest hits, ties aside):
-//  * every distinct BLAS rebuilt with binned SAH, leaves of up to 4 triangles;
+//  * every distinct BLAS rebuilt with binned SAH, binary leaves of up to
+//    `blas_leaf` triangles (the wide collapse merges them into leaves of up to
+//    CG_LEAF_MAX where that is cheaper);
 //  * instances whose inverse transform is the identity and whose BLAS holds at
 //    most 16 triangles (e.g. one-triangle emitters) merged into one world-space
 //    BLAS whose triangles carry their instance index -- their local rays equal
@@ -684,7 +686,7 @@ struct BlasJob {
 };
 
 inline int build_sah_accel(const cg_scene_data* sd, Layout& L, std::string& err,
-                           std::vector<BlasJob>* jobs = nullptr) {
+                           std::vector<BlasJob>* jobs = nullptr, uint32_t blas_leaf = 2u) {
   const uint32_t nb = sd->blas_node_count, ni = sd->instance_count, nt = sd->tlas_node_count;
   // World boxes of the instances: the reference TLAS leaves.
   std::vector<int> have_box(ni, 0);
@@ -833,7 +835,7 @@ inline int build_sah_accel(const cg_scene_data* sd, Layout& L, std::string& err,
         continue;
       }
       SahTree t;
-      t.build(prims, CG_LEAF_MAX);
+      t.build(prims, blas_leaf);
       const uint32_t slot_base = static_cast<uint32_t>(L.tris.size());
       append(&t.order);
       built_root[ri] = 0;  // placeholder until pair nodes are placed
@@ -881,7 +883,7 @@ inline int build_sah_accel(const cg_scene_data* sd, Layout& L, std::string& err,
       }
     } else {
       SahTree t;
-      t.build(prims, CG_LEAF_MAX);
+      t.build(prims, blas_leaf);
       const uint32_t slot_base = static_cast<uint32_t>(L.tris.size());
       for (uint32_t id : t.order) {
         L.tris.push_back(make_tri(sd, tri_inst[id].first, tri_inst[id].second));
@@ -1005,6 +1007,170 @@ struct CollapseWide {
   }
 };
 
+// 8-wide BVH from a binary one by the SAH-optimal collapse (Ylitie, Karras and
+// Laine, "Efficient Incoherent Ray Traversal on GPUs Through Compressed Wide
+// BVHs", HPG 2017), priced for this traversal: a node visit is one step, a
+// leaf one step per two triangles, entering an instance one step, each
+// weighted by the area of the box whose hit makes the ray take it.  For a
+// binary node n and a budget of i = 1..8 slots, Entry::cost[i - 1] is the
+// least cost of n's subtree given as at most i children of the wide node
+// above: n itself (a wide node, or one leaf when its subtree is at most
+// CG_LEAF_MAX triangles in consecutive slots), or n's two subtrees sharing the
+// i slots.  Every child box is a box of the binary tree, bit for bit; nodes
+// are stored parent first, depth first.
+struct CollapseWideSah {
+  static constexpr float kNode = 1.0f, kTriStep = 1.0f, kInst = 1.0f;
+  struct Entry {
+    float cost[CG_WIDE];
+    uint8_t kl[CG_WIDE], kr[CG_WIDE];  // the subtrees' budgets; both 0: n itself
+    uint8_t il, ir;                    // the subtrees' budgets when n is a wide node
+    uint8_t leaf, done;                // n itself is one leaf / entry computed
+    uint32_t first, count;             // the subtree's slot run (count 0: none)
+    float area;                        // half area of n's box (held by its parent)
+  };
+  const std::vector<PairNode>& pairs;
+  std::vector<Node8>& out;
+  std::vector<Entry> e;
+  std::vector<uint32_t> map;
+  CollapseWideSah(const std::vector<PairNode>& p, std::vector<Node8>& o)
+      : pairs(p), out(o), e(p.size()), map(p.size(), CG_REF_NONE) {
+    for (Entry& x : e) x.done = 0;
+  }
+  static bool interior(uint32_t r) { return CollapseWide::interior(r); }
+  static float leaf_steps(uint32_t count) { return static_cast<float>((count + 1u) / 2u); }
+  float child_cost(uint32_t r, const float* lo, const float* hi, int budget) const {
+    if (interior(r)) return e[r].cost[budget - 1];
+    if (r == CG_REF_NONE) return 0.0f;
+    const float a = CollapseWide::half_area(lo, hi);
+    if ((r & CG_REF_LEAF) != 0u) return a * kTriStep * leaf_steps(((r >> CG_REF_COUNT_SHIFT) & 3u) + 1u);
+    return a * kInst;
+  }
+  void run_of(uint32_t r, uint32_t& first, uint32_t& count) const {
+    first = 0u;
+    count = 0u;
+    if (interior(r)) {
+      first = e[r].first;
+      count = e[r].count;
+    } else if (r != CG_REF_NONE && (r & CG_REF_LEAF) != 0u) {
+      first = r & CG_REF_SLOT_MASK;
+      count = ((r >> CG_REF_COUNT_SHIFT) & 3u) + 1u;
+    }
+  }
+  void evaluate(uint32_t n) {
+    const PairNode& p = pairs[n];
+    Entry& x = e[n];
+    uint32_t fl, cl, fr, cr;
+    run_of(p.ref_l, fl, cl);
+    run_of(p.ref_r, fr, cr);
+    x.first = 0u;
+    x.count = 0u;
+    if (cl != 0u && cr != 0u && cl + cr <= CG_LEAF_MAX) {
+      if (fl + cl == fr) {
+        x.first = fl;
+        x.count = cl + cr;
+      } else if (fr + cr == fl) {
+        x.first = fr;
+        x.count = cl + cr;
+      }
+    }
+    float dist[CG_WIDE + 1];
+    uint8_t dl[CG_WIDE + 1], dr[CG_WIDE + 1];
+    for (int j = 2; j <= CG_WIDE; ++j) {
+      dist[j] = INFINITY;
+      for (int k = 1; k < j; ++k) {
+        const float c = child_cost(p.ref_l, p.lo_l, p.hi_l, k) + child_cost(p.ref_r, p.lo_r, p.hi_r, j - k);
+        if (c < dist[j]) {
+          dist[j] = c;
+          dl[j] = static_cast<uint8_t>(k);
+          dr[j] = static_cast<uint8_t>(j - k);
+        }
+      }
+    }
+    const float internal = x.area * kNode + dist[CG_WIDE];
+    const float leaf = x.count != 0u ? x.area * kTriStep * leaf_steps(x.count) : INFINITY;
+    x.il = dl[CG_WIDE];
+    x.ir = dr[CG_WIDE];
+    x.leaf = leaf <= internal ? 1u : 0u;
+    x.cost[0] = std::min(internal, leaf);
+    x.kl[0] = x.kr[0] = 0u;
+    for (int i = 2; i <= CG_WIDE; ++i) {
+      if (dist[i] < x.cost[i - 2]) {
+        x.cost[i - 1] = dist[i];
+        x.kl[i - 1] = dl[i];
+        x.kr[i - 1] = dr[i];
+      } else {
+        x.cost[i - 1] = x.cost[i - 2];
+        x.kl[i - 1] = x.kl[i - 2];
+        x.kr[i - 1] = x.kr[i - 2];
+      }
+    }
+    x.done = 1u;
+  }
+  // Children before parents, from one root (an explicit stack: binary trees
+  // over one triangle per leaf run deep).
+  void solve(uint32_t root) {
+    if (!interior(root) || e[root].done) return;
+    std::vector<std::pair<uint32_t, bool>> st{{root, false}};
+    while (!st.empty()) {
+      const auto [n, expanded] = st.back();
+      st.pop_back();
+      if (e[n].done) continue;
+      if (expanded) {
+        evaluate(n);
+        continue;
+      }
+      st.push_back({n, true});
+      const PairNode& p = pairs[n];
+      if (interior(p.ref_l)) {
+        e[p.ref_l].area = CollapseWide::half_area(p.lo_l, p.hi_l);
+        if (!e[p.ref_l].done) st.push_back({p.ref_l, false});
+      }
+      if (interior(p.ref_r)) {
+        e[p.ref_r].area = CollapseWide::half_area(p.lo_r, p.hi_r);
+        if (!e[p.ref_r].done) st.push_back({p.ref_r, false});
+      }
+    }
+  }
+  void gather(uint32_t r, const float* lo, const float* hi, int budget, CollapseWide::Child* ch, int& n) const {
+    if (interior(r)) {
+      const Entry& x = e[r];
+      if (x.kl[budget - 1] != 0u) {
+        const PairNode& p = pairs[r];
+        gather(p.ref_l, p.lo_l, p.hi_l, x.kl[budget - 1], ch, n);
+        gather(p.ref_r, p.lo_r, p.hi_r, x.kr[budget - 1], ch, n);
+        return;
+      }
+      if (x.leaf) r = CG_MAKE_LEAF(x.first, x.count);
+    }
+    ch[n++] = CollapseWide::Child{r, {lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}};
+  }
+  uint32_t convert(uint32_t r) {
+    if (!interior(r)) return r;
+    if (map[r] != CG_REF_NONE) return map[r];
+    solve(r);
+    CollapseWide::Child ch[CG_WIDE];
+    int n = 0;
+    const PairNode& p = pairs[r];
+    gather(p.ref_l, p.lo_l, p.hi_l, e[r].il, ch, n);
+    gather(p.ref_r, p.lo_r, p.hi_r, e[r].ir, ch, n);
+    const uint32_t idx = static_cast<uint32_t>(out.size());
+    map[r] = idx;
+    out.emplace_back();
+    Node8 nd;
+    for (int i = 0; i < CG_WIDE; ++i) {
+      const bool used = i < n;
+      for (int a = 0; a < 3; ++a) {
+        nd.lo[a][i] = used ? ch[i].lo[a] : __builtin_inff();
+        nd.hi[a][i] = used ? ch[i].hi[a] : -__builtin_inff();
+      }
+      nd.ref[i] = CG_REF_NONE;
+    }
+    for (int i = 0; i < n; ++i) nd.ref[i] = convert(ch[i].ref);
+    out[idx] = nd;
+    return idx;
+  }
+};
+
 // Deepest traversal stack of a wide tree: a node with k children pushes at
 // most k - 1 of them and descends into the nearest; entering an instance
 // pushes one marker.
@@ -1028,12 +1194,15 @@ inline uint32_t stack_bound_wide(const std::vector<Node8>& bvh, const std::vecto
   return memo[r] = (k > 0 ? k - 1u : 0u) + deepest;
 }
 
+template <class Collapse = CollapseWideSah>
 inline void collapse_scene_bvh(Layout& L) {
   L.bvh.clear();
   L.bvh.reserve(L.nodes.size() / 4 + 1);
-  CollapseWide c(L.nodes, L.bvh);
-  for (InstRec& in : L.insts) in.root = c.convert(in.root);
-  L.root = c.convert(L.root);
+  {
+    Collapse c(L.nodes, L.bvh);
+    for (InstRec& in : L.insts) in.root = c.convert(in.root);
+    L.root = c.convert(L.root);
+  }
   std::vector<uint32_t> memo(L.bvh.size(), ~0u);
   L.stack_depth = stack_bound_wide(L.bvh, L.insts, L.root, memo);
   L.nodes.clear();
