@@ -89,6 +89,10 @@ struct cg_env {
   void* scratch = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress)
   DevBuf errflag;           // device error flag: a traversal stack overflowed
   DevBuf samples;           // sample-stream mode: per-sample results of the last render
+  DevBuf sched;             // sample-stream mode: item durations, the next order, the sort's buffers
+  uint32_t sched_items = 0; // work items the schedule buffers are sized for
+  size_t sched_tmp = 0;     // the sort's temporary bytes
+  bool sched_ready = false; // the order holds a complete render's durations
   uint32_t rng_mode = CG_RNG_REFERENCE;
   int test_stack_cap = -1;  // CG_TEST_STACK_CAP: caps overflow entries (tests force an overflow)
   std::vector<cg_batch*> batches;  // live wavefront batches, destroyed with the env
@@ -418,9 +422,29 @@ __device__ __forceinline__ SampleTask sample_task(uint32_t tile, uint32_t c, uin
 // are nchunks work items of 64 lane tasks (sample_task), so a pixel's samples
 // run on separate lanes and no lane carries a long chain of dependent
 // queries; the per-sample results go to B and k_view_finish sums them.
+// An item's duration (wall-clock ticks) as a sort key: four steps per octave,
+// at most 127, so one 8-bit radix pass orders the items and items of one step
+// keep their image order.
+__device__ __forceinline__ uint32_t duration_key(uint64_t ticks) {
+  const uint32_t t = ticks > 0xFFFFFFFFull ? 0xFFFFFFFFu : static_cast<uint32_t>(ticks);
+  if (t < 4u) return t;
+  const uint32_t e = 31u - static_cast<uint32_t>(__clz(t));
+  return (e << 2) | ((t >> (e - 2u)) & 3u);
+}
+// The position in the longest-first order that interleaved queue slot s
+// takes: XCD x's slots x, x + 8, x + 16, ... walk runs of kDealRun
+// consecutive positions, so each XCD takes neighbouring items of one duration
+// step (mostly one tile's samples) together and its L2 sees them together.
+constexpr uint32_t kDealRun = 16u;
+__device__ __forceinline__ uint32_t dealt(uint32_t s, uint32_t total) {
+  constexpr uint32_t g = kXcds * kDealRun;
+  if (s >= total / g * g) return s;  // the last partial group in queue order
+  return s / g * g + (s % kXcds) * kDealRun + (s % g) / kXcds;
+}
 template <int MF>
 __global__ __launch_bounds__(kBlock, 5) void k_view_s(ViewArgs A, SampleBufs B, uint32_t chunk, uint32_t nchunks,
-                                                      WorkQueue q, StackArgs sa) {
+                                                      WorkQueue q, StackArgs sa, const uint32_t* __restrict__ order,
+                                                      uint32_t* __restrict__ duration) {
   __shared__ uint2 s_stk[kSampleShortStack * kBlock];
   smits_lds_init();
   const TStack stk = lane_stack<kBlock>(s_stk, sa, kSampleShortStack);
@@ -430,11 +454,14 @@ __global__ __launch_bounds__(kBlock, 5) void k_view_s(ViewArgs A, SampleBufs B, 
   uint32_t probe = 0;
   const uint32_t lane = threadIdx.x & 63u;
   for (;;) {
-    const uint32_t item = grab_item(q, probe);
-    if (item == kNoItem) break;
+    const uint32_t slot = grab_item(q, probe);
+    if (slot == kNoItem) break;
+    const uint32_t item = order ? order[dealt(slot, q.total)] : slot;
+    const uint64_t t0 = wall_clock64();
     const uint32_t tile = item / nchunks, c = item - tile * nchunks;
     const SampleTask k = sample_task(tile, c, lane, tiles_x, nchunks, chunk, A.spp);
     if (k.px < A.W && k.py < A.H) render_samples<MF>(A, B, k.px, k.py, k.s0, k.s1, stk, st, nrays, nshadow);
+    if (duration && lane == 0u) duration[item] = duration_key(static_cast<uint64_t>(wall_clock64()) - t0);
   }
   if (A.stats) flush_stats(A.stats, nrays, nshadow, st.nodes, st.tris, st.insts);
 }
@@ -1015,6 +1042,69 @@ int launch_view_ref(cg_env* env, const ViewArgs& a, int frames, uint32_t tiles, 
   if (frames <= 16) return launch_view_mf<16>(env, a, tiles, depth);
   return launch_view_mf<64>(env, a, tiles, depth);
 }
+// Sample-stream schedule.  A render of the same size as the last one takes
+// its work items longest first by the last render's durations (progressive
+// frames of one view repeat the same work), so its long items start early and
+// the frame does not end waiting on one of them; the first render, or one of
+// another size, takes them in image order.  The order decides which wave runs
+// an item, never its result (every sample draws from its own stream).
+// Buffers: durations, the order, the sorted keys, the identity, sort storage.
+struct Schedule {
+  uint32_t* duration = nullptr;
+  const uint32_t* order = nullptr;  // null: image order
+  uint32_t* next = nullptr;         // the order the sort writes for the next render
+  uint32_t* keys = nullptr;
+  uint32_t* ids = nullptr;
+  void* tmp = nullptr;
+};
+__global__ void k_iota_u32(uint32_t* p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = i;
+}
+int schedule(cg_env* env, uint32_t items, Schedule& sc) {
+  const size_t head = (4u * static_cast<size_t>(items) * sizeof(uint32_t) + 255u) / 256u * 256u;
+  if (env->sched_items != items) {
+    if (env->sched.p) {
+      CG_HIP(env, hipStreamSynchronize(env->stream));
+      dev_free(env->sched);
+    }
+    env->sched_items = 0;
+    env->sched_ready = false;
+    size_t tmp = 0;
+    if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, static_cast<const uint32_t*>(nullptr),
+                                                     static_cast<uint32_t*>(nullptr),
+                                                     static_cast<const uint32_t*>(nullptr),
+                                                     static_cast<uint32_t*>(nullptr), static_cast<int>(items), 0, 8,
+                                                     env->stream) != hipSuccess)
+      return fail(env, CG_ERR_HIP, "sample-stream schedule: sort sizing failed");
+    int rc = dev_alloc(env, env->sched, head + tmp);
+    if (rc != CG_OK) return rc;
+    env->sched_items = items;
+    env->sched_tmp = tmp;
+    hipLaunchKernelGGL(k_iota_u32, dim3((items + kBlock - 1u) / kBlock), dim3(kBlock), 0, env->stream,
+                       static_cast<uint32_t*>(env->sched.p) + 3u * static_cast<size_t>(items), items);
+  }
+  uint32_t* base = static_cast<uint32_t*>(env->sched.p);
+  const size_t n = items;
+  sc.duration = base;
+  sc.next = base + n;
+  sc.order = env->sched_ready ? sc.next : nullptr;
+  sc.keys = base + 2u * n;
+  sc.ids = base + 3u * n;
+  sc.tmp = static_cast<char*>(env->sched.p) + head;
+  return CG_OK;
+}
+// After the render (same stream): the next order, longest first; items of
+// equal key keep their image order (the sort is stable).
+int reorder(cg_env* env, uint32_t items, const Schedule& sc) {
+  size_t tmp = env->sched_tmp;
+  if (hipcub::DeviceRadixSort::SortPairsDescending(sc.tmp, tmp, sc.duration, sc.keys, sc.ids, sc.next,
+                                                   static_cast<int>(items), 0, 8,
+                                                   env->stream) != hipSuccess)
+    return fail(env, CG_ERR_HIP, "sample-stream schedule: sort failed");
+  env->sched_ready = true;
+  return CG_OK;
+}
 // Samples per lane of the sample-stream mode: one up to 64 samples per pixel
 // (a 64-spp wave is one pixel's samples), then enough that a tile has at
 // most 64 items (the image does not depend on it: samples are summed in
@@ -1029,11 +1119,16 @@ int launch_view_s_mf(cg_env* env, const ViewArgs& a, const SampleBufs& b, uint32
   StackArgs sa;
   int rc = stack_args(env, grid, depth, sa, kSampleShortStack);
   if (rc != CG_OK) return rc;
+  Schedule sc;
+  rc = schedule(env, static_cast<uint32_t>(items), sc);
+  if (rc != CG_OK) return rc;
   WorkQueue q = reset_queue(env, static_cast<uint32_t>(items));
-  hipLaunchKernelGGL((k_view_s<MF>), dim3(grid), dim3(kBlock), 0, env->stream, a, b, chunk, nchunks, q, sa);
+  q.interleave = sc.order ? 1u : 0u;  // the longest items dealt round-robin to the XCDs
+  hipLaunchKernelGGL((k_view_s<MF>), dim3(grid), dim3(kBlock), 0, env->stream, a, b, chunk, nchunks, q, sa, sc.order,
+                     sc.duration);
   const uint32_t npix = a.W * a.H;
   hipLaunchKernelGGL(k_view_finish, dim3((npix + kBlock - 1u) / kBlock), dim3(kBlock), 0, env->stream, a, b);
-  return CG_OK;
+  return reorder(env, static_cast<uint32_t>(items), sc);
 }
 // The sample-stream buffers of a render: W*H*spp XYZ values (and path records).
 int sample_bufs(cg_env* env, const ViewArgs& a, SampleBufs& b) {
@@ -1236,6 +1331,7 @@ int cg_env_destroy(cg_env* env) {
   dev_free(env->ovf);
   dev_free(env->errflag);
   dev_free(env->samples);
+  dev_free(env->sched);
   for (cg_vbatch* v : env->vbatch_all) {
     if (v->stream) (void)hipStreamSynchronize(v->stream);
     for (DevBuf* d : {&v->jobs, &v->first, &v->queue, &v->ovf, &v->jobs_s, &v->first_s, &v->pfirst_s, &v->samples})
