@@ -19,8 +19,8 @@ untimed check `matched_rmse` measures each frame's relative MSE against a
 converged image (65 accumulated sample-stream frames, the measured frame's
 own samples taken out) and reports the ratio.  The timed frames are frames
 of one progressive view, each taking its work items longest first by the
-previous frame's durations; `sample_stream_schedule` times a fresh view's
-first frame (image order) beside its second.
+previous frame's durations; `sample_stream_schedule` times the same frames
+in image order beside them.
 
 Prints one JSON line.  --gpus N > 1 is rejected: this path runs on one GPU.
 """
@@ -649,24 +649,24 @@ def main() -> int:
     ms_s = ev.elapsed_ms(e0, e1) / args.steps
 
     # The timed frames take their work items longest first by the previous
-    # frame's durations (the env's sample-stream schedule).  A fresh env's
-    # first frame has no durations yet and takes them in image order: timed
-    # here beside its second frame, as the cost of a view's first frame.
-    with cg.DeviceEnv(conf) as env2:
-        ds2 = env2.upload(sd)
-        env2.set_rng_mode(env2.RNG_SAMPLE)
-        env2.sync()
-        marks = [ev.create() for _ in range(3)]
-        ev.record(marks[0], env2.stream)
-        env2.view(ds2, cam, 0)
-        ev.record(marks[1], env2.stream)
-        env2.view(ds2, cam, 0)
-        ev.record(marks[2], env2.stream)
-        env2.sync()
-        cold = {"first_frame_ms": round(ev.elapsed_ms(marks[0], marks[1]), 3),
-                "second_frame_ms": round(ev.elapsed_ms(marks[1], marks[2]), 3),
-                "note": "a fresh env's first frame takes the work items in image order; later frames of the same "
-                        "size take them longest first by the previous frame's durations (images identical)"}
+    # frame's durations (the env's sample-stream schedule).  The same frames
+    # with the schedule off take them in image order, as a view's first frame
+    # does: timed here for comparison.
+    env.set_schedule(False)
+    for _ in range(args.warmup):
+        env.view(ds, cam, 0)
+    env.sync()
+    e0, e1 = ev.create(), ev.create()
+    ev.record(e0, stream)
+    for _ in range(args.steps):
+        env.view(ds, cam, 0)
+    ev.record(e1, stream)
+    env.sync()
+    env.set_schedule(True)
+    cold = {"longest_first_ms": round(ms_s, 3), "image_order_ms": round(ev.elapsed_ms(e0, e1) / args.steps, 3),
+            "note": "the timed frames take the work items longest first by the previous frame's durations; a "
+                    "view's first frame, or every frame with cg_env_set_schedule off, takes them in image order "
+                    "(images identical)"}
 
     # Matched rMSE (untimed): the 16-spp frame of each RNG order against a
     # converged image.  Frames 0..64 of the sample-stream mode accumulate a

@@ -249,6 +249,10 @@ class DeviceEnv:
         """RNG_REFERENCE: the reference's one stream per pixel; RNG_SAMPLE: one stream per sample (cg_device.h)."""
         self._check(native.device().cg_env_set_rng_mode(self._h, mode))
 
+    def set_schedule(self, on: bool = True) -> None:
+        """Sample-stream work order: longest first by the last render's durations (on), or image order."""
+        self._check(native.device().cg_env_set_schedule(self._h, int(on)))
+
     def set_counting(self, on: bool = True) -> None:
         self._check(native.device().cg_env_set_counting(self._h, int(on)))
 

@@ -93,6 +93,7 @@ struct cg_env {
   uint32_t sched_items = 0; // work items the schedule buffers are sized for
   size_t sched_tmp = 0;     // the sort's temporary bytes
   bool sched_ready = false; // the order holds a complete render's durations
+  bool sched_on = true;     // cg_env_set_schedule
   uint32_t rng_mode = CG_RNG_REFERENCE;
   int test_stack_cap = -1;  // CG_TEST_STACK_CAP: caps overflow entries (tests force an overflow)
   std::vector<cg_batch*> batches;  // live wavefront batches, destroyed with the env
@@ -1062,6 +1063,7 @@ __global__ void k_iota_u32(uint32_t* p, uint32_t n) {
   if (i < n) p[i] = i;
 }
 int schedule(cg_env* env, uint32_t items, Schedule& sc) {
+  if (!env->sched_on) return CG_OK;  // image order, nothing recorded
   const size_t head = (4u * static_cast<size_t>(items) * sizeof(uint32_t) + 255u) / 256u * 256u;
   if (env->sched_items != items) {
     if (env->sched.p) {
@@ -1097,6 +1099,7 @@ int schedule(cg_env* env, uint32_t items, Schedule& sc) {
 // After the render (same stream): the next order, longest first; items of
 // equal key keep their image order (the sort is stable).
 int reorder(cg_env* env, uint32_t items, const Schedule& sc) {
+  if (!sc.duration) return CG_OK;
   size_t tmp = env->sched_tmp;
   if (hipcub::DeviceRadixSort::SortPairsDescending(sc.tmp, tmp, sc.duration, sc.keys, sc.ids, sc.next,
                                                    static_cast<int>(items), 0, 8,
@@ -1379,6 +1382,13 @@ int cg_env_set_rng_mode(cg_env* env, int mode) {
   if (!env) return fail(nullptr, CG_ERR_INVALID, "null env");
   if (mode != CG_RNG_REFERENCE && mode != CG_RNG_SAMPLE) return fail(env, CG_ERR_INVALID, "unknown rng mode %d", mode);
   env->rng_mode = static_cast<uint32_t>(mode);
+  return CG_OK;
+}
+
+int cg_env_set_schedule(cg_env* env, int enable) {
+  if (!env) return fail(nullptr, CG_ERR_INVALID, "null env");
+  env->sched_on = enable != 0;
+  env->sched_ready = false;  // durations are recorded again from the next render
   return CG_OK;
 }
 

@@ -244,6 +244,7 @@ def device() -> C.CDLL:
         _sig(d, "cg_stats_read", I32, P, C.POINTER(Stats))
         _sig(d, "cg_env_set_accel", I32, P, I32)
         _sig(d, "cg_env_set_rng_mode", I32, P, I32)
+        _sig(d, "cg_env_set_schedule", I32, P, I32)
         _sig(d, "cg_stats_reset", I32, P)
         _sig(d, "cg_scene_upload", I32, P, C.POINTER(SceneDataC), C.POINTER(P))
         _sig(d, "cg_scene_refit", I32, P, P, C.POINTER(SceneDataC))

@@ -4,7 +4,8 @@ durations; that decides which wave runs an item, never its result.  Bar:
 every frame of a progressive sequence equals, bit for bit (XYZ history,
 carried seed, path records), the same frame rendered by a fresh environment
 from the same history in image order, including after a render of another
-size has reset the schedule."""
+size has reset the schedule and with the schedule switched off
+(cg_env_set_schedule)."""
 import numpy as np
 import pytest
 
@@ -36,7 +37,9 @@ def test_scheduled_frames_equal_image_order(cg, name, params, w, h, spp, depth):
         env.set_rng_mode(env.RNG_SAMPLE)
         env.set_path_records(True)
         prev = None
-        for it in range(4):
+        for it in range(5):
+            if it == 4:  # the schedule switched off: image order
+                env.set_schedule(False)
             if it == 2:  # a render of another size in this env resets the schedule
                 job = cg.RenderJob(env, ds, cam, cg.Conf.make(w // 2, h // 2, spp, depth), rng_mode=1)
                 job.view_alone()
