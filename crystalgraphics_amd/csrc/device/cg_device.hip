@@ -89,11 +89,13 @@ struct cg_env {
   void* scratch = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress)
   DevBuf errflag;           // device error flag: a traversal stack overflowed
   DevBuf samples;           // sample-stream mode: per-sample results of the last render
-  DevBuf sched;             // sample-stream mode: item durations, the next order, the sort's buffers
-  uint32_t sched_items = 0; // work items the schedule buffers are sized for
-  size_t sched_tmp = 0;     // the sort's temporary bytes
-  bool sched_ready = false; // the order holds a complete render's durations
-  bool sched_on = true;     // cg_env_set_schedule
+  // Work schedules of the per-pixel renders ([0] reference order, [1]
+  // sample-stream mode): item durations, the next order, the sort's buffers.
+  DevBuf sched[2];
+  uint32_t sched_items[2] = {0u, 0u};  // work items the buffers are sized for
+  size_t sched_tmp[2] = {0u, 0u};      // the sort's temporary bytes
+  bool sched_ready[2] = {false, false};  // the order holds a complete render's durations
+  bool sched_on = true;                  // cg_env_set_schedule
   uint32_t rng_mode = CG_RNG_REFERENCE;
   int test_stack_cap = -1;  // CG_TEST_STACK_CAP: caps overflow entries (tests force an overflow)
   std::vector<cg_batch*> batches;  // live wavefront batches, destroyed with the env
@@ -373,6 +375,25 @@ __global__ __launch_bounds__(kBlock, 5) void k_trace(DevScene S, const cg_ray* _
   if (stats) flush_stats(stats, kAnyHit ? 0u : nq, kAnyHit ? nq : 0u, st.nodes, st.tris, st.insts);
 }
 
+// An item's duration (wall-clock ticks) as a sort key: four steps per octave,
+// at most 127, so one 8-bit radix pass orders the items and items of one step
+// keep their image order.
+__device__ __forceinline__ uint32_t duration_key(uint64_t ticks) {
+  const uint32_t t = ticks > 0xFFFFFFFFull ? 0xFFFFFFFFu : static_cast<uint32_t>(ticks);
+  if (t < 4u) return t;
+  const uint32_t e = 31u - static_cast<uint32_t>(__clz(t));
+  return (e << 2) | ((t >> (e - 2u)) & 3u);
+}
+// The position in the longest-first order that interleaved queue slot s
+// takes: XCD x's slots x, x + 8, x + 16, ... walk runs of kDealRun
+// consecutive positions, so each XCD takes neighbouring items of one duration
+// step (mostly one tile's samples) together and its L2 sees them together.
+constexpr uint32_t kDealRun = 16u;
+__device__ __forceinline__ uint32_t dealt(uint32_t s, uint32_t total) {
+  constexpr uint32_t g = kXcds * kDealRun;
+  if (s >= total / g * g) return s;  // the last partial group in queue order
+  return s / g * g + (s % kXcds) * kDealRun + (s % g) / kXcds;
+}
 // ---------------------------------------------------------------------------
 // The compute pass (compute.wesl:32-62, render.wesl:7-18, trace.wesl:22-113)
 // with one lane per pixel, so each pixel consumes its RNG stream in exactly
@@ -380,7 +401,9 @@ __global__ __launch_bounds__(kBlock, 5) void k_trace(DevScene S, const cg_ray* _
 // in a private array (MF frames = trace_depth - 1); the BVH stack in LDS
 // (overflow in device memory).
 template <int MF>
-__global__ __launch_bounds__(kBlock, 3) void k_view(ViewArgs A, WorkQueue q, StackArgs sa) {
+__global__ __launch_bounds__(kBlock, 3) void k_view(ViewArgs A, WorkQueue q, StackArgs sa,
+                                                    const uint32_t* __restrict__ order,
+                                                    uint32_t* __restrict__ duration) {
   __shared__ uint2 s_stk[kViewShortStack * kBlock];
   smits_lds_init();
   const TStack stk = lane_stack<kBlock>(s_stk, sa, kViewShortStack);
@@ -390,11 +413,14 @@ __global__ __launch_bounds__(kBlock, 3) void k_view(ViewArgs A, WorkQueue q, Sta
   uint32_t probe = 0;
   const uint32_t lane = threadIdx.x & 63u;
   for (;;) {
-    const uint32_t tile = grab_item(q, probe);
-    if (tile == kNoItem) break;
+    const uint32_t slot = grab_item(q, probe);
+    if (slot == kNoItem) break;
+    const uint32_t tile = order ? order[dealt(slot, q.total)] : slot;
+    const uint64_t t0 = wall_clock64();
     const uint32_t px = (tile % tiles_x) * 8u + (lane & 7u);
     const uint32_t py = (tile / tiles_x) * 8u + (lane >> 3);
     if (px < A.W && py < A.H) render_pixel<MF>(A, px, py, stk, st, nrays, nshadow);
+    if (duration && lane == 0u) duration[tile] = duration_key(static_cast<uint64_t>(wall_clock64()) - t0);
   }
   if (A.stats) flush_stats(A.stats, nrays, nshadow, st.nodes, st.tris, st.insts);
 }
@@ -423,25 +449,6 @@ __device__ __forceinline__ SampleTask sample_task(uint32_t tile, uint32_t c, uin
 // are nchunks work items of 64 lane tasks (sample_task), so a pixel's samples
 // run on separate lanes and no lane carries a long chain of dependent
 // queries; the per-sample results go to B and k_view_finish sums them.
-// An item's duration (wall-clock ticks) as a sort key: four steps per octave,
-// at most 127, so one 8-bit radix pass orders the items and items of one step
-// keep their image order.
-__device__ __forceinline__ uint32_t duration_key(uint64_t ticks) {
-  const uint32_t t = ticks > 0xFFFFFFFFull ? 0xFFFFFFFFu : static_cast<uint32_t>(ticks);
-  if (t < 4u) return t;
-  const uint32_t e = 31u - static_cast<uint32_t>(__clz(t));
-  return (e << 2) | ((t >> (e - 2u)) & 3u);
-}
-// The position in the longest-first order that interleaved queue slot s
-// takes: XCD x's slots x, x + 8, x + 16, ... walk runs of kDealRun
-// consecutive positions, so each XCD takes neighbouring items of one duration
-// step (mostly one tile's samples) together and its L2 sees them together.
-constexpr uint32_t kDealRun = 16u;
-__device__ __forceinline__ uint32_t dealt(uint32_t s, uint32_t total) {
-  constexpr uint32_t g = kXcds * kDealRun;
-  if (s >= total / g * g) return s;  // the last partial group in queue order
-  return s / g * g + (s % kXcds) * kDealRun + (s % g) / kXcds;
-}
 template <int MF>
 __global__ __launch_bounds__(kBlock, 5) void k_view_s(ViewArgs A, SampleBufs B, uint32_t chunk, uint32_t nchunks,
                                                       WorkQueue q, StackArgs sa, const uint32_t* __restrict__ order,
@@ -1027,29 +1034,15 @@ int stack_args(cg_env* env, uint32_t grid, uint32_t depth, StackArgs& sa, int sh
   sa.ovf_cap = cap;
   return CG_OK;
 }
-template <int MF>
-int launch_view_mf(cg_env* env, const ViewArgs& a, uint32_t tiles, uint32_t depth) {
-  const uint32_t grid = resident_grid(k_view<MF>, env, (tiles + 3u) / 4u);
-  StackArgs sa;
-  int rc = stack_args(env, grid, depth, sa, kViewShortStack);
-  if (rc != CG_OK) return rc;
-  WorkQueue q = reset_queue(env, tiles);
-  hipLaunchKernelGGL((k_view<MF>), dim3(grid), dim3(kBlock), 0, env->stream, a, q, sa);
-  return CG_OK;
-}
-int launch_view_ref(cg_env* env, const ViewArgs& a, int frames, uint32_t tiles, uint32_t depth) {
-  if (frames <= 4) return launch_view_mf<4>(env, a, tiles, depth);
-  if (frames <= 8) return launch_view_mf<8>(env, a, tiles, depth);
-  if (frames <= 16) return launch_view_mf<16>(env, a, tiles, depth);
-  return launch_view_mf<64>(env, a, tiles, depth);
-}
-// Sample-stream schedule.  A render of the same size as the last one takes
-// its work items longest first by the last render's durations (progressive
-// frames of one view repeat the same work), so its long items start early and
-// the frame does not end waiting on one of them; the first render, or one of
-// another size, takes them in image order.  The order decides which wave runs
-// an item, never its result (every sample draws from its own stream).
-// Buffers: durations, the order, the sorted keys, the identity, sort storage.
+// Work schedule of the per-pixel renders (one per kernel: k = 0 reference
+// order, k = 1 sample-stream mode).  A render of the same size as the
+// kernel's last one takes its work items longest first by that render's
+// durations (progressive frames of one view repeat the same work), so its
+// long items start early and the frame does not end waiting on one of them;
+// the first render, or one of another size, takes them in image order.  The
+// order decides which wave runs an item, never its result (each pixel or
+// sample draws from its own stream).  Buffers: durations, the order, the
+// sorted keys, the identity, sort storage.
 struct Schedule {
   uint32_t* duration = nullptr;
   const uint32_t* order = nullptr;  // null: image order
@@ -1062,51 +1055,73 @@ __global__ void k_iota_u32(uint32_t* p, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = i;
 }
-int schedule(cg_env* env, uint32_t items, Schedule& sc) {
+int schedule(cg_env* env, int k, uint32_t items, Schedule& sc) {
   if (!env->sched_on) return CG_OK;  // image order, nothing recorded
+  size_t tmp = 0;
+  if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, static_cast<const uint32_t*>(nullptr),
+                                                   static_cast<uint32_t*>(nullptr),
+                                                   static_cast<const uint32_t*>(nullptr),
+                                                   static_cast<uint32_t*>(nullptr), static_cast<int>(items), 0, 8,
+                                                   env->stream) != hipSuccess)
+    return fail(env, CG_ERR_HIP, "render schedule: sort sizing failed");
   const size_t head = (4u * static_cast<size_t>(items) * sizeof(uint32_t) + 255u) / 256u * 256u;
-  if (env->sched_items != items) {
-    if (env->sched.p) {
+  if (head + tmp > env->sched[k].bytes) {  // the buffers only grow: renders of many sizes reuse them
+    if (env->sched[k].p) {
       CG_HIP(env, hipStreamSynchronize(env->stream));
-      dev_free(env->sched);
+      dev_free(env->sched[k]);
     }
-    env->sched_items = 0;
-    env->sched_ready = false;
-    size_t tmp = 0;
-    if (hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, static_cast<const uint32_t*>(nullptr),
-                                                     static_cast<uint32_t*>(nullptr),
-                                                     static_cast<const uint32_t*>(nullptr),
-                                                     static_cast<uint32_t*>(nullptr), static_cast<int>(items), 0, 8,
-                                                     env->stream) != hipSuccess)
-      return fail(env, CG_ERR_HIP, "sample-stream schedule: sort sizing failed");
-    int rc = dev_alloc(env, env->sched, head + tmp);
+    env->sched_items[k] = 0;
+    int rc = dev_alloc(env, env->sched[k], head + tmp);
     if (rc != CG_OK) return rc;
-    env->sched_items = items;
-    env->sched_tmp = tmp;
-    hipLaunchKernelGGL(k_iota_u32, dim3((items + kBlock - 1u) / kBlock), dim3(kBlock), 0, env->stream,
-                       static_cast<uint32_t*>(env->sched.p) + 3u * static_cast<size_t>(items), items);
   }
-  uint32_t* base = static_cast<uint32_t*>(env->sched.p);
+  uint32_t* base = static_cast<uint32_t*>(env->sched[k].p);
   const size_t n = items;
+  if (env->sched_items[k] != items) {  // another size: image order this time, and a new identity
+    env->sched_ready[k] = false;
+    env->sched_items[k] = items;
+    hipLaunchKernelGGL(k_iota_u32, dim3((items + kBlock - 1u) / kBlock), dim3(kBlock), 0, env->stream, base + 3u * n,
+                       items);
+  }
+  env->sched_tmp[k] = tmp;
   sc.duration = base;
   sc.next = base + n;
-  sc.order = env->sched_ready ? sc.next : nullptr;
+  sc.order = env->sched_ready[k] ? sc.next : nullptr;
   sc.keys = base + 2u * n;
   sc.ids = base + 3u * n;
-  sc.tmp = static_cast<char*>(env->sched.p) + head;
+  sc.tmp = static_cast<char*>(env->sched[k].p) + head;
   return CG_OK;
 }
 // After the render (same stream): the next order, longest first; items of
 // equal key keep their image order (the sort is stable).
-int reorder(cg_env* env, uint32_t items, const Schedule& sc) {
+int reorder(cg_env* env, int k, uint32_t items, const Schedule& sc) {
   if (!sc.duration) return CG_OK;
-  size_t tmp = env->sched_tmp;
+  size_t tmp = env->sched_tmp[k];
   if (hipcub::DeviceRadixSort::SortPairsDescending(sc.tmp, tmp, sc.duration, sc.keys, sc.ids, sc.next,
                                                    static_cast<int>(items), 0, 8,
                                                    env->stream) != hipSuccess)
-    return fail(env, CG_ERR_HIP, "sample-stream schedule: sort failed");
-  env->sched_ready = true;
+    return fail(env, CG_ERR_HIP, "render schedule: sort failed");
+  env->sched_ready[k] = true;
   return CG_OK;
+}
+template <int MF>
+int launch_view_mf(cg_env* env, const ViewArgs& a, uint32_t tiles, uint32_t depth) {
+  const uint32_t grid = resident_grid(k_view<MF>, env, (tiles + 3u) / 4u);
+  StackArgs sa;
+  int rc = stack_args(env, grid, depth, sa, kViewShortStack);
+  if (rc != CG_OK) return rc;
+  Schedule sc;
+  rc = schedule(env, 0, tiles, sc);
+  if (rc != CG_OK) return rc;
+  WorkQueue q = reset_queue(env, tiles);
+  q.interleave = sc.order ? 1u : 0u;  // the longest tiles dealt to the XCDs in runs
+  hipLaunchKernelGGL((k_view<MF>), dim3(grid), dim3(kBlock), 0, env->stream, a, q, sa, sc.order, sc.duration);
+  return reorder(env, 0, tiles, sc);
+}
+int launch_view_ref(cg_env* env, const ViewArgs& a, int frames, uint32_t tiles, uint32_t depth) {
+  if (frames <= 4) return launch_view_mf<4>(env, a, tiles, depth);
+  if (frames <= 8) return launch_view_mf<8>(env, a, tiles, depth);
+  if (frames <= 16) return launch_view_mf<16>(env, a, tiles, depth);
+  return launch_view_mf<64>(env, a, tiles, depth);
 }
 // Samples per lane of the sample-stream mode: one up to 64 samples per pixel
 // (a 64-spp wave is one pixel's samples), then enough that a tile has at
@@ -1123,7 +1138,7 @@ int launch_view_s_mf(cg_env* env, const ViewArgs& a, const SampleBufs& b, uint32
   int rc = stack_args(env, grid, depth, sa, kSampleShortStack);
   if (rc != CG_OK) return rc;
   Schedule sc;
-  rc = schedule(env, static_cast<uint32_t>(items), sc);
+  rc = schedule(env, 1, static_cast<uint32_t>(items), sc);
   if (rc != CG_OK) return rc;
   WorkQueue q = reset_queue(env, static_cast<uint32_t>(items));
   q.interleave = sc.order ? 1u : 0u;  // the longest items dealt round-robin to the XCDs
@@ -1131,7 +1146,7 @@ int launch_view_s_mf(cg_env* env, const ViewArgs& a, const SampleBufs& b, uint32
                      sc.duration);
   const uint32_t npix = a.W * a.H;
   hipLaunchKernelGGL(k_view_finish, dim3((npix + kBlock - 1u) / kBlock), dim3(kBlock), 0, env->stream, a, b);
-  return reorder(env, static_cast<uint32_t>(items), sc);
+  return reorder(env, 1, static_cast<uint32_t>(items), sc);
 }
 // The sample-stream buffers of a render: W*H*spp XYZ values (and path records).
 int sample_bufs(cg_env* env, const ViewArgs& a, SampleBufs& b) {
@@ -1334,7 +1349,8 @@ int cg_env_destroy(cg_env* env) {
   dev_free(env->ovf);
   dev_free(env->errflag);
   dev_free(env->samples);
-  dev_free(env->sched);
+  dev_free(env->sched[0]);
+  dev_free(env->sched[1]);
   for (cg_vbatch* v : env->vbatch_all) {
     if (v->stream) (void)hipStreamSynchronize(v->stream);
     for (DevBuf* d : {&v->jobs, &v->first, &v->queue, &v->ovf, &v->jobs_s, &v->first_s, &v->pfirst_s, &v->samples})
@@ -1388,7 +1404,7 @@ int cg_env_set_rng_mode(cg_env* env, int mode) {
 int cg_env_set_schedule(cg_env* env, int enable) {
   if (!env) return fail(nullptr, CG_ERR_INVALID, "null env");
   env->sched_on = enable != 0;
-  env->sched_ready = false;  // durations are recorded again from the next render
+  env->sched_ready[0] = env->sched_ready[1] = false;  // durations are recorded again from the next render
   return CG_OK;
 }
 
