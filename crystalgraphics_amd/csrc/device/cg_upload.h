@@ -1077,6 +1077,8 @@ struct CollapseWideSah {
     uint8_t dl[CG_WIDE + 1], dr[CG_WIDE + 1];
     for (int j = 2; j <= CG_WIDE; ++j) {
       dist[j] = INFINITY;
+      dl[j] = 1u;  // a valid split even when every cost overflows or is NaN
+      dr[j] = static_cast<uint8_t>(j - 1);
       for (int k = 1; k < j; ++k) {
         const float c = child_cost(p.ref_l, p.lo_l, p.hi_l, k) + child_cost(p.ref_r, p.lo_r, p.hi_r, j - k);
         if (c < dist[j]) {
@@ -1090,7 +1092,7 @@ struct CollapseWideSah {
     const float leaf = x.count != 0u ? x.area * kTriStep * leaf_steps(x.count) : INFINITY;
     x.il = dl[CG_WIDE];
     x.ir = dr[CG_WIDE];
-    x.leaf = leaf <= internal ? 1u : 0u;
+    x.leaf = x.count != 0u && leaf <= internal ? 1u : 0u;  // (both costs may be infinite)
     x.cost[0] = std::min(internal, leaf);
     x.kl[0] = x.kr[0] = 0u;
     for (int i = 2; i <= CG_WIDE; ++i) {

@@ -181,3 +181,34 @@ def test_quantised_nodes_contain_exact_boxes(cg, name, params, accel):
     rc = native.device().cg_layout_check(C.byref(sd.c), accel, C.byref(n), C.byref(depth))
     assert rc == 0, native.device().cg_last_error()
     assert (n.value > 0 or name == "quad") and 1 <= depth.value <= 128  # quad: one instance, one leaf
+
+
+def test_layout_of_huge_coordinates(cg):
+    """Triangles spread over 1e20 units: the collapse's surface-area costs
+    overflow to infinity in float, and the build must still produce a valid
+    8-wide tree (the containment and one-leaf-per-slot checks of
+    cg_layout_check)."""
+    from crystalgraphics_amd import native
+    rng = np.random.default_rng(4)
+    n = 300
+    base = rng.uniform(-1e20, 1e20, (n, 1, 3))
+    pos = (base + rng.uniform(-1e18, 1e18, (n, 3, 3))).reshape(-1, 3)
+    v = np.zeros(len(pos), cg.VERTEX)
+    v["pos"] = pos
+    v["nrm"] = [0, 0, 1]
+    m = np.zeros(1, cg.MATERIAL)
+    m["base_color_factor"] = [0.5, 0.5, 0.5, 1]
+    m["roughness_factor"] = 1
+    m["ior"] = 1.5
+    m["tex"]["index"] = -1
+    p = np.zeros(1, cg.PRIMITIVE)
+    p["vertex_count"] = len(pos)
+    p["index_count"] = len(pos)
+    p["world"] = np.eye(4, dtype=np.float32).reshape(-1)
+    p["inv_world"] = np.eye(4, dtype=np.float32).reshape(-1)
+    s = cg.HostScene.from_arrays(v, np.arange(len(pos), dtype=np.uint32), m, p, tag=93)
+    sd = s.scene_data()
+    nodes, depth = C.c_uint64(), C.c_uint32()
+    rc = native.device().cg_layout_check(C.byref(sd.c), 1, C.byref(nodes), C.byref(depth))
+    assert rc == 0, native.device().cg_last_error()
+    assert nodes.value > 0 and 1 <= depth.value <= 128
