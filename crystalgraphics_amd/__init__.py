@@ -250,7 +250,7 @@ class DeviceEnv:
         self._check(native.device().cg_env_set_rng_mode(self._h, mode))
 
     def set_schedule(self, on: bool = True) -> None:
-        """Sample-stream work order: longest first by the last render's durations (on), or image order."""
+        """Work order of view/view_job: longest first by the last same-size render's durations (on), or image order."""
         self._check(native.device().cg_env_set_schedule(self._h, int(on)))
 
     def set_counting(self, on: bool = True) -> None:

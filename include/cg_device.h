@@ -105,11 +105,12 @@ int cg_env_set_accel(cg_env* env, int mode);
  *     reference-order ones. */
 enum { CG_RNG_REFERENCE = 0, CG_RNG_SAMPLE = 1 };
 int cg_env_set_rng_mode(cg_env* env, int mode);
-/* Work order of the sample-stream mode (no counterpart in the reference, a
+/* Work order of cg_view and cg_view_job (no counterpart in the reference, a
  * scheduling switch for callers that time or compare orders).  On (default):
- * a render of the same size as the env's last sample-stream render takes its
- * work items longest first by that render's durations; off: every render
- * takes them in image order.  Either way the results are identical. */
+ * a render of the same size as the env's last render in the same RNG mode
+ * takes its work items (tiles, or a tile's samples in chunks) longest first
+ * by that render's durations; off: every render takes them in image order.
+ * Either way the results are identical. */
 int cg_env_set_schedule(cg_env* env, int enable);
 int cg_stats_reset(cg_env* env);
 
