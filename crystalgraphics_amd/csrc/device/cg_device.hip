@@ -880,10 +880,10 @@ void place_tlas(Layout& L, uint32_t base) {
 int lbvh_build_device(cg_env* env, BlasJob& j, uint32_t node_base, const TriRec* tris_src, const uint4* vidx_src,
                       TriRec* tris, uint4* vidx, DevBuf& nodes, BlasState& keep) {
   const uint32_t n = static_cast<uint32_t>(j.boxes.size());
-  DevBuf prims, keys, keys2, ids, ids2, left, right, first, count, parent, box, flag, fa, fb, ctr, tmp, kids;
+  DevBuf prims, keys, keys2, ids, ids2, left, right, first, count, parent, box, flag, fa, fb, ctr, tmp, kids, price;
   auto release = [&] {
     for (DevBuf* b : {&prims, &keys, &keys2, &ids, &ids2, &left, &right, &first, &count, &parent, &box, &flag, &fa,
-                      &fb, &ctr, &tmp, &kids})
+                      &fb, &ctr, &tmp, &kids, &price})
       dev_free(*b);
   };
   auto grid = [](uint32_t items) { return dim3((items + kBlock - 1u) / kBlock); };
@@ -896,6 +896,7 @@ int lbvh_build_device(cg_env* env, BlasJob& j, uint32_t node_base, const TriRec*
   for (DevBuf* b : {&left, &right, &first, &count, &flag}) alloc(*b, (n - 1u) * sizeof(uint32_t));
   alloc(parent, (2u * n - 1u) * sizeof(uint32_t));
   alloc(box, (n - 1u) * sizeof(PrimBox));
+  alloc(price, (n - 1u) * sizeof(Price));
   alloc(fa, (n - 1u) * sizeof(uint2));
   alloc(fb, (n - 1u) * sizeof(uint2));
   alloc(ctr, 4 * sizeof(uint32_t));
@@ -927,6 +928,7 @@ int lbvh_build_device(cg_env* env, BlasJob& j, uint32_t node_base, const TriRec*
   A.parent = static_cast<uint32_t*>(parent.p);
   A.box = static_cast<PrimBox*>(box.p);
   A.flag = static_cast<uint32_t*>(flag.p);
+  A.price = static_cast<Price*>(price.p);
   uint32_t* c = static_cast<uint32_t*>(ctr.p);  // [0] next frontier length, [1] wide nodes, [2] most children
   A.front_out_n = c;
   A.wide_n = c + 1;

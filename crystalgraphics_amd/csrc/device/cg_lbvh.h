@@ -1,10 +1,11 @@
 // cg_lbvh.h — building a BLAS on the device: a linear BVH (Karras, "Maximizing
 // Parallelism in the Construction of BVHs, Octrees, and k-d Trees", HPG 2012)
-// over 30-bit Morton codes of the triangle box centres, bottom-up boxes,
-// then a top-down collapse into 8-wide nodes (a node opens its child of
-// largest surface area until it has eight; subtrees of at most CG_LEAF_MAX
-// triangles become leaves, which are contiguous ranges of the sorted
-// triangles) quantised as the host build's nodes are.
+// over 30-bit Morton codes of the triangle box centres, bottom-up boxes and
+// collapse prices, then a top-down collapse into 8-wide nodes (the
+// SAH-optimal collapse the host build uses: each node's children follow the
+// prices' choices; subtrees of at most CG_LEAF_MAX triangles may become
+// leaves, which are contiguous ranges of the sorted triangles) quantised as
+// the host build's nodes are.
 //
 // Each step is a per-thread function over plain arrays: the kernels in
 // cg_device.hip run them one thread per item, and lbvh_build_host runs the
@@ -21,6 +22,19 @@ namespace cg {
 
 struct PrimBox {
   float lo[3], hi[3];
+};
+
+// The SAH-optimal collapse's price of an internal node n (Ylitie, Karras and
+// Laine, HPG 2017; the host build's CollapseWideSah): cost[i - 1] is the least
+// cost of n's subtree given as at most i children of the wide node above, a
+// node visit costing one step and a leaf one step per two triangles, each
+// weighted by its box's half area.  kl/kr: the two subtrees' budgets (both 0:
+// n itself, a wide node or, with `leaf`, one leaf); il/ir: their budgets
+// when n is a wide node.
+struct Price {
+  float cost[CG_WIDE];
+  uint8_t kl[CG_WIDE], kr[CG_WIDE];
+  uint8_t il, ir, leaf, pad;
 };
 
 // Child encoding in the binary tree: internal node index, or leaf (sorted
@@ -42,6 +56,7 @@ struct LbvhArrays {
   uint32_t* parent;           // n - 1 internal + n leaves (leaf i at n - 1 + i)
   PrimBox* box;               // n - 1 internal node boxes (padded)
   uint32_t* flag;             // n - 1 arrival counters, zero before the bounds pass
+  Price* price;               // n - 1 collapse prices (null: boxes only, as a refit)
   // collapse
   uint2* front_in;            // (binary internal node, wide node) pairs
   uint2* front_out;
@@ -154,13 +169,65 @@ CG_QHD uint32_t lbvh_arrive(uint32_t* flag) {
   return (*flag)++;
 #endif
 }
+CG_QHD float lbvh_half_area(const PrimBox& b) {
+  const float x = b.hi[0] - b.lo[0], y = b.hi[1] - b.lo[1], z = b.hi[2] - b.lo[2];
+  return x * y + y * z + z * x;
+}
+CG_QHD float lbvh_child_price(const LbvhArrays& A, uint32_t c, int budget) {
+  if (c & kLbvhLeaf) return lbvh_half_area(lbvh_leaf_box(A, c & ~kLbvhLeaf));  // one triangle: one step
+  return A.price[c].cost[budget - 1];
+}
+// The price of internal node p, from its box and its children's prices.
+CG_QHD void lbvh_price(const LbvhArrays& A, uint32_t p) {
+  const uint32_t l = A.left[p], r = A.right[p];
+  float dist[CG_WIDE + 1];
+  uint8_t dl[CG_WIDE + 1], dr[CG_WIDE + 1];
+  for (int j = 2; j <= CG_WIDE; ++j) {
+    dist[j] = __builtin_inff();
+    dl[j] = 1u;  // a valid split even when every cost overflows
+    dr[j] = static_cast<uint8_t>(j - 1);
+    for (int k = 1; k < j; ++k) {
+      const float c = lbvh_child_price(A, l, k) + lbvh_child_price(A, r, j - k);
+      if (c < dist[j]) {
+        dist[j] = c;
+        dl[j] = static_cast<uint8_t>(k);
+        dr[j] = static_cast<uint8_t>(j - k);
+      }
+    }
+  }
+  Price x;
+  const float area = lbvh_half_area(A.box[p]);
+  const uint32_t cnt = A.count[p];
+  const float internal = area + dist[CG_WIDE];
+  const bool can_leaf = cnt <= CG_LEAF_MAX;
+  const float leaf = can_leaf ? area * static_cast<float>((cnt + 1u) / 2u) : __builtin_inff();
+  x.il = dl[CG_WIDE];
+  x.ir = dr[CG_WIDE];
+  x.leaf = can_leaf && leaf <= internal ? 1u : 0u;
+  x.pad = 0u;
+  x.cost[0] = fminf(internal, leaf);
+  x.kl[0] = x.kr[0] = 0u;
+  for (int i = 2; i <= CG_WIDE; ++i) {
+    if (dist[i] < x.cost[i - 2]) {
+      x.cost[i - 1] = dist[i];
+      x.kl[i - 1] = dl[i];
+      x.kr[i - 1] = dr[i];
+    } else {
+      x.cost[i - 1] = x.cost[i - 2];
+      x.kl[i - 1] = x.kl[i - 2];
+      x.kr[i - 1] = x.kr[i - 2];
+    }
+  }
+  A.price[p] = x;
+}
 CG_QHD void lbvh_bounds(const LbvhArrays& A, uint32_t leaf) {
   uint32_t p = A.parent[A.n - 1u + leaf];
   // a Karras tree is at most 62 levels deep (30 key bits, 32 position bits)
   for (int level = 0; p != kLbvhNone && level < 64; ++level) {
     if (lbvh_arrive(&A.flag[p]) == 0u) return;
-    // the release half of the next node's arrival publishes this box
+    // the release half of the next node's arrival publishes this box and price
     A.box[p] = lbvh_union(lbvh_child_box(A, A.left[p]), lbvh_child_box(A, A.right[p]));
+    if (A.price) lbvh_price(A, p);
     p = A.parent[p];
   }
 }
@@ -175,32 +242,33 @@ CG_QHD uint32_t lbvh_alloc(uint32_t* counter) {
 #endif
 }
 CG_QHD void lbvh_collapse(const LbvhArrays& A, uint2 entry) {
+  // The children: the node's two subtrees with the budgets its price chose,
+  // each given as itself or split again by its own price.  The budgets on the
+  // stack and the children taken sum to at most eight, so neither overflows.
   uint32_t ch[CG_WIDE];
   PrimBox bx[CG_WIDE];
-  int n = 2;
-  ch[0] = A.left[entry.x];
-  ch[1] = A.right[entry.x];
-  bx[0] = lbvh_child_box(A, ch[0]);
-  bx[1] = lbvh_child_box(A, ch[1]);
-  auto openable = [&](uint32_t c) { return (c & kLbvhLeaf) == 0u && A.count[c] > CG_LEAF_MAX; };
-  while (n < CG_WIDE) {
-    int best = -1;
-    float ba = -1.0f;
-    for (int k = 0; k < n; ++k)
-      if (openable(ch[k])) {
-        const float x = bx[k].hi[0] - bx[k].lo[0], y = bx[k].hi[1] - bx[k].lo[1], z = bx[k].hi[2] - bx[k].lo[2];
-        const float a = x * y + y * z + z * x;
-        if (a > ba) {
-          ba = a;
-          best = k;
-        }
-      }
-    if (best < 0) break;
-    const uint32_t c = ch[best];
-    ch[best] = A.left[c];
-    bx[best] = lbvh_child_box(A, ch[best]);
-    ch[n] = A.right[c];
-    bx[n] = lbvh_child_box(A, ch[n]);
+  int n = 0;
+  uint32_t sn[CG_WIDE];
+  uint8_t sb[CG_WIDE];
+  int sp = 0;
+  const Price& root = A.price[entry.x];
+  sn[sp] = A.right[entry.x];
+  sb[sp++] = root.ir;
+  sn[sp] = A.left[entry.x];
+  sb[sp++] = root.il;
+  while (sp > 0) {
+    const uint32_t c = sn[--sp];
+    const int b = sb[sp];
+    if ((c & kLbvhLeaf) == 0u && A.price[c].kl[b - 1] != 0u) {
+      const Price& pc = A.price[c];
+      sn[sp] = A.right[c];
+      sb[sp++] = pc.kr[b - 1];
+      sn[sp] = A.left[c];
+      sb[sp++] = pc.kl[b - 1];
+      continue;
+    }
+    ch[n] = c;
+    bx[n] = lbvh_child_box(A, c);
     ++n;
   }
   Node8 nd;
@@ -215,7 +283,7 @@ CG_QHD void lbvh_collapse(const LbvhArrays& A, uint2 entry) {
     const uint32_t c = ch[k];
     if (c & kLbvhLeaf) {
       nd.ref[k] = CG_MAKE_LEAF(A.slot_base + (c & ~kLbvhLeaf), 1u);
-    } else if (A.count[c] <= CG_LEAF_MAX) {
+    } else if (A.price[c].leaf) {
       nd.ref[k] = CG_MAKE_LEAF(A.slot_base + A.first[c], A.count[c]);
     } else {
       const uint32_t w = lbvh_alloc(A.wide_n);

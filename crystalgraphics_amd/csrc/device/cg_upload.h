@@ -1251,6 +1251,7 @@ inline std::vector<uint32_t> lbvh_build_host(BlasJob& j, uint32_t node_base, std
   std::vector<uint32_t> keys(n), ids(n), left(n - 1), right(n - 1), first(n - 1), count(n - 1),
       parent(2 * n - 1, kLbvhNone), flag(n - 1, 0u);
   std::vector<PrimBox> box(n - 1);
+  std::vector<Price> price(n - 1);
   std::vector<Node8Q> wq(n - 1);
   std::vector<Node8> wx(n - 1);
   std::vector<uint2> front{make_uint2(0u, 0u)}, next;
@@ -1269,6 +1270,7 @@ inline std::vector<uint32_t> lbvh_build_host(BlasJob& j, uint32_t node_base, std
   A.parent = parent.data();
   A.box = box.data();
   A.flag = flag.data();
+  A.price = price.data();
   A.front_out_n = &next_n;
   A.wide_n = &wide_n;
   A.max_children = &maxc;
